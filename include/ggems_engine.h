@@ -1,0 +1,172 @@
+/*
+ * ggems_engine.h -- thin C ABI of the MI355X photon-transport engine.
+ *
+ * This is the inner drop-in boundary: the entry points GGEMS's host code
+ * would call in place of its OpenCL launch sites.  Plain C types only
+ * (pointers, sizes, POD descriptors); every call returns 0 on success or a
+ * negative code, with a message from ggh_last_error().  Nothing throws.
+ *
+ * Reference launch sites each entry point replaces:
+ *   ggh_set_physics       GGEMSCrossSections::Initialize + GGEMSMaterials::BuildMaterialTables
+ *                         uploads (src/physics/GGEMSCrossSections.cc:170-219,
+ *                         src/materials/GGEMSMaterials.cc:209-300)
+ *   ggh_add_voxel_solid   GGEMSVoxelizedSolid::LoadVolumeImage / UpdateTransformationMatrix
+ *                         (src/geometries/GGEMSVoxelizedSolid.cc:217-317)
+ *   ggh_add_box_solid     GGEMSSolidBox constructor + histogram allocation
+ *                         (src/geometries/GGEMSSolidBox.cc:40-112)
+ *   ggh_set_dosimetry     GGEMSDosimetryCalculator::Initialize
+ *                         (src/navigators/GGEMSDosimetryCalculator.cc:435-531)
+ *   ggh_set_xray_source   GGEMSXRaySource::Initialize / FillEnergy
+ *                         (src/sources/GGEMSXRaySource.cc:241-256, src/sources/GGEMSSource.cc:265-316)
+ *   ggh_seed              GGEMSPseudoRandomGenerator::InitializeSeeds
+ *                         (src/randoms/GGEMSPseudoRandomGenerator.cc:140-167)
+ *   ggh_run               GGEMS::RunOnDevice batch/hop loop: get_primaries, particle_solid_distance_*,
+ *                         project_to_*, track_through_*, is_alive
+ *                         (src/global/GGEMS.cc:303-363, src/navigators/GGEMSNavigator.cc:320-530,
+ *                          src/sources/GGEMSXRaySource.cc:100-148, src/physics/GGEMSParticles.cc:142-193)
+ *   ggh_compute_dose      GGEMSDosimetryCalculator::ComputeDose
+ *                         (src/navigators/GGEMSDosimetryCalculator.cc:346-430)
+ *   ggh_get_histogram     GGEMSSystem::SaveResults readback (src/navigators/GGEMSSystem.cc:165-256)
+ *   ggh_get_dose_tallies  GGEMSDosimetryCalculator::Save* readback
+ *                         (src/navigators/GGEMSDosimetryCalculator.cc:536-809)
+ */
+#ifndef GGEMS_ENGINE_H
+#define GGEMS_ENGINE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GGH_MAX_SLOTS 1048576   /* particle slots per device (MAXIMUM_PARTICLES) */
+#define GGH_MAX_SOLIDS 256
+#define GGH_MAX_HOPS 100        /* navigator hops per primary (src/global/GGEMS.cc:330) */
+
+typedef struct ggh_context ggh_context;
+
+/* Oriented box: local->global 4x4 matrix, row major (GGfloat44 m0_..m3_),
+ * and the local-frame borders. */
+typedef struct {
+  float matrix[16];
+  float border_min[3];
+  float border_max[3];
+} ggh_obb;
+
+/* Physics tables for every material of every navigator, concatenated.
+ * A navigator's material i is global material (material_offset + i). */
+typedef struct {
+  int32_t nbins;                 /* energy bins of the cross-section tables */
+  int32_t n_materials;           /* global material count */
+  int32_t n_processes;           /* activated photon processes */
+  int32_t process_id[3];         /* activation order: 0 Compton, 1 photoelectric, 2 Rayleigh */
+  const float* energy_bins;      /* [nbins] MeV */
+  const float* cross_sections;   /* [3][n_materials][nbins] mm^-1 (rows of inactive processes ignored) */
+  const float* cs_atom_rayleigh; /* [101][nbins] mm^2 (Rayleigh per atom, element selection) */
+  int32_t total_elements;
+  const int32_t* n_elements;     /* [n_materials] */
+  const int32_t* element_index;  /* [n_materials] offset into the element arrays */
+  const uint8_t* atomic_number;  /* [total_elements] */
+  const float* atom_density;     /* [total_elements] atoms/mm^3 */
+  const float* photon_cut;       /* [n_materials] MeV */
+  const float* density;          /* [n_materials] internal density units */
+  const float* rayleigh_pp;      /* [9][101] form-factor fit parameters PP0..PP8 */
+  float rayleigh_factor;         /* 0.5*(hc)^-2 constant of the angular sampler */
+} ggh_physics_desc;
+
+typedef struct {
+  ggh_obb obb;
+  int32_t material_offset;
+  int32_t n_voxels[3];
+  float voxel_size[3];
+  const uint8_t* labels;         /* host pointer, n_voxels[0]*n_voxels[1]*n_voxels[2] bytes */
+} ggh_voxel_solid_desc;
+
+typedef struct {
+  ggh_obb obb;
+  int32_t material_offset;
+  int32_t n_elements[3];         /* virtual detection elements */
+  float box_size[3];
+  float threshold;               /* MeV */
+  int32_t store_scatter;
+} ggh_box_solid_desc;
+
+typedef struct {
+  int32_t solid;                 /* voxel solid index returned by ggh_add_voxel_solid */
+  float dosel_size[3];
+  int32_t n_dosels[3];
+  float border_min[3];
+  float border_max[3];
+  int32_t edep_squared;          /* keep sum of edep^2 */
+  int32_t hit;                   /* keep hit counts */
+  int32_t photon_tracking;       /* keep photon fluence counts */
+} ggh_dosimetry_desc;
+
+typedef struct {
+  float matrix[16];              /* source local->global */
+  float aperture;                /* rad */
+  float focal_spot[3];           /* mm */
+  int32_t n_energy;              /* spectrum lines; energy/cdf hold n_energy+1 entries */
+  const float* energy;
+  const float* cdf;
+  int32_t is_interp;
+} ggh_xray_source_desc;
+
+/* run statistics (cumulative since the last ggh_reset_tallies) */
+typedef struct {
+  uint64_t primaries;
+  uint64_t voxel_steps;
+  uint64_t interactions;         /* discrete processes inside voxel solids */
+  uint64_t detector_steps;
+  uint64_t detector_interactions;/* discrete processes inside box solids */
+  uint64_t hops;
+} ggh_stats;
+
+const char* ggh_last_error(void);
+int ggh_device_count(int* count);
+
+int ggh_context_create(int device, ggh_context** ctx);
+int ggh_context_destroy(ggh_context* ctx);
+
+int ggh_set_physics(ggh_context* ctx, const ggh_physics_desc* desc);
+/* both return the new solid's index (>= 0) or a negative error; solids are
+ * visited in the order they are added, as GGEMS registers them */
+int ggh_add_voxel_solid(ggh_context* ctx, const ggh_voxel_solid_desc* desc);
+int ggh_add_box_solid(ggh_context* ctx, const ggh_box_solid_desc* desc);
+int ggh_set_dosimetry(ggh_context* ctx, const ggh_dosimetry_desc* desc);
+int ggh_set_xray_source(ggh_context* ctx, const ggh_xray_source_desc* desc);
+
+/* seeds every particle slot from std::mt19937(seed); seed 0 draws from /dev/urandom */
+int ggh_seed(ggh_context* ctx, uint32_t seed);
+/* transports n primaries (batches of GGH_MAX_SLOTS, slot s owns RNG stream s);
+ * tallies accumulate across calls */
+int ggh_run(ggh_context* ctx, uint64_t n_particles);
+int ggh_synchronize(ggh_context* ctx);
+/* milliseconds of device time of the last ggh_run transport kernel */
+int ggh_last_kernel_ms(ggh_context* ctx, float* ms);
+int ggh_reset_tallies(ggh_context* ctx);
+int ggh_get_stats(ggh_context* ctx, ggh_stats* out);
+
+int ggh_get_histogram(ggh_context* ctx, int solid, int32_t* counts, int32_t* scatter);
+int ggh_get_dose_tallies(ggh_context* ctx, double* edep, double* edep_squared, int32_t* hit,
+                         int32_t* photon_tracking);
+int ggh_compute_dose(ggh_context* ctx, float scale_factor, int32_t water_reference, float minimum_density,
+                     float* dose, float* uncertainty);
+
+/* ---- test entry points -------------------------------------------------- */
+/* voxel indices visited by rays crossing voxel solid `solid` with no
+ * interaction; pos/dir are global.  out_index[r*max_steps + k], out_count[r]. */
+int ggh_debug_voxel_walk(ggh_context* ctx, int solid, int n_rays, const float* pos, const float* dir,
+                         int max_steps, int32_t* out_index, int32_t* out_count);
+/* the first primary of each of the first n slots (consumes their RNG) */
+int ggh_debug_primaries(ggh_context* ctx, int n, float* pos, float* dir, float* energy);
+/* n draws from slot 0..n_slots-1 streams: out[s*n + k] */
+int ggh_debug_uniforms(ggh_context* ctx, int n_slots, int n, float* out);
+/* host implementation of the engine's uniform mapping (exact-equivalence checks) */
+void ggh_host_uniform_from_sum(const uint32_t* sums, float* out, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
