@@ -1,0 +1,165 @@
+#!/usr/bin/env python3
+"""Primary photons/s of the MI355X engine on the voxel-CT workload.
+
+Workload (one GPU): the 512x512x256 body phantom of 0.8 mm voxels (air, lung,
+adipose, soft tissue, cortical bone; 40 seeded ellipsoids), a point cone-beam
+source at SID 600 mm with a 120 kVp Kramers spectrum (2.5 mm Al), a flat
+256x256 detector of 1 mm GOS pixels at SDD 1000 mm, Compton + photoelectric +
+Rayleigh, FP64 dose + uncertainty tallies.  Set up through the GGEMS API
+(ggems_amd.configs.voxel_ct), then timed on the engine with every input
+resident in HBM: one step transports --photons-per-step primaries
+(batches of 1,048,576 particle slots, as GGEMS runs them).
+
+Prints one JSON line (metric, value, roofline, cpu_baseline, ...).
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "primary photons/s on 1 MI355X (voxel CT); scatter error vs 1e9-photon MC"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes_per_primary(stats):
+    """B = S*1 (uint8 label gather) + I*40 (FP64 edep + edep^2 and int32 hit
+    read-modify-writes) + D*8 (int32 histogram RMW) + 48 (source / final state)."""
+    p = max(stats["primaries"], 1)
+    s = stats["voxel_steps"] / p
+    i = stats["interactions"] / p
+    d = stats["detector_interactions"] / p
+    return s * 1.0 + i * 40.0 + d * 8.0 + 48.0, s, i, d
+
+
+def cpu_baseline(workdir, n_per_step, seed, target_s, threads):
+    """The CPU oracle on a bounded sample of particle slots of the same workload."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import math
+
+    import numpy as np
+    from scene import Scene
+
+    from ggems_amd import authoring, configs
+    mhd, rng_file, spec = configs.voxel_ct_inputs(workdir)
+    lab, spacing = authoring.read_mhd(mhd)
+    e, w = np.loadtxt(spec, unpack=True)
+    sc = Scene()
+    sc.add_phantom(lab, authoring.BODY_MATERIALS, tuple(spacing), dosimetry={"photon_tracking": False})
+    sc.add_ct_flat("GOS", (1, 1), (256, 256, 1), (1.0, 1.0, 1.0), 600.0, 1000.0, 10e-3)
+    sc.set_source((-600.0, 0.0, 0.0), math.degrees(math.atan(128 * math.sqrt(2.0) / 1000.0)), e, intensities=w)
+    sc.build()
+    prepared = sc.prepare_oracle()  # tallies allocated and faulted in outside the timed region
+    pilot = 2048
+    t0 = time.perf_counter()
+    r = sc.run_oracle(n_per_step, seed, threads=threads, slots=(0, pilot), prepared=prepared)
+    dt = time.perf_counter() - t0
+    rate = r["stats"]["primaries"] / dt
+    per_slot = r["stats"]["primaries"] / pilot
+    slots = int(min(max(pilot, target_s * rate / per_slot), 1048576 - pilot))
+    t0 = time.perf_counter()
+    r = sc.run_oracle(n_per_step, seed, threads=threads, slots=(pilot, pilot + slots), prepared=prepared)
+    dt = time.perf_counter() - t0
+    return {
+        "value": r["stats"]["primaries"] / dt,
+        "unit": "photons/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "CPU oracle (oracle/, C restatement of the reference kernels), slots %d..%d of a %d-primary "
+                  "voxel-CT step: %d primaries in %.1f s" % (pilot, pilot + slots, n_per_step,
+                                                             r["stats"]["primaries"], dt),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--photons-per-step", type=int, default=1 << 24)
+    ap.add_argument("--seed", type=int, default=777)
+    ap.add_argument("--workdir", default=os.path.join(REPO, "ggems_amd", "data", "_gen", "voxel_ct"))
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU; every box here has one MI355X "
+              "(multi-GPU sharding of the photon budget is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from ggems_amd import configs
+    from ggems_amd.ggems import GGEMS
+
+    keep = configs.voxel_ct(args.workdir, n_particles=args.photons_per_step, seed=20240, dosimetry=True)
+    g = GGEMS()
+    g.initialize(args.seed)
+    eng = g.engine()
+    for _ in range(args.warmup):
+        eng.run(args.photons_per_step)
+    eng.reset_tallies()
+    torch.cuda.synchronize()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.run(args.photons_per_step)
+        kernel_ms.append(eng.last_kernel_ms())
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    stats = eng.stats()
+    total = args.steps * args.photons_per_step
+    assert stats["primaries"] == total, stats
+    value = total / wall
+    bpp, s, i, d = algorithmic_bytes_per_primary(stats)
+    avg_kernel_s = sum(kernel_ms) / len(kernel_ms) * 1e-3
+    achieved = bpp * args.photons_per_step / avg_kernel_s / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "photons/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "voxel CT: 512x512x256 body phantom (0.8 mm, 5 materials, 40 seeded ellipsoids), "
+                        "120 kVp Kramers + 2.5 mm Al point cone beam at SID 600 mm, flat 256x256 GOS detector "
+                        "(1 mm) at SDD 1000 mm, Compton+PE+Rayleigh, FP64 dose+uncertainty",
+            "photons_per_step": args.photons_per_step,
+            "voxel_steps_per_primary": s,
+            "interactions_per_primary": i,
+            "detector_interactions_per_primary": d,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "transport_kernel",
+            "kernel_ms": avg_kernel_s * 1e3,
+            "bytes_per_primary": bpp,
+        },
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.workdir, args.photons_per_step, args.seed, args.cpu_seconds,
+                                           args.cpu_threads)
+    g.delete()
+    del keep
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

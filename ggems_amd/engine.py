@@ -153,10 +153,21 @@ class Engine:
         self.box_elements = {}
         self.n_dosels = None
 
+    @classmethod
+    def attach(cls, handle):
+        """Non-owning view of a context created elsewhere (e.g. by libggems.so)."""
+        if not handle:
+            raise EngineError("no engine context (GGEMS not initialised)")
+        e = cls.__new__(cls)
+        e._keep, e.solid_kinds, e.box_elements, e.n_dosels = [], [], {}, None
+        e.h = C.c_void_p(handle)
+        e._owned = False
+        return e
+
     def close(self):
-        if self.h:
+        if self.h and getattr(self, "_owned", True):
             library().ggh_context_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

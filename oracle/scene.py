@@ -1,4 +1,4 @@
-"""Scene builders shared by the parity tests.
+"""Scene builders for the oracle (TEST INFRASTRUCTURE ONLY).
 
 A Scene holds one voxelized phantom (optional dosimetry), CT modules and an
 X-ray cone-beam source, with physics tables built by the CPU oracle.  The
@@ -19,7 +19,7 @@ import sys
 import numpy as np
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-sys.path.insert(0, os.path.join(REPO, "oracle"))
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, REPO)
 
 import ggems_oracle as O  # noqa: E402
@@ -234,9 +234,21 @@ class Scene:
         n = [int(F(pay["bmax"][i] - pay["bmin"][i]) / size[i]) for i in range(3)]
         return dict(size=size, n=n)
 
-    def run_oracle(self, n, seed, threads=0, slots=None):
+    def prepare_oracle(self):
+        """Builds the oracle structs once, with every tally page faulted in
+        (a steady-state run keeps its tallies allocated)."""
         tallies = {}
-        ph, solids, src = self._oracle_structs(tallies)
+        structs = self._oracle_structs(tallies)
+        for k, v in tallies.items():
+            for a in (v if isinstance(v, list) else [v]):
+                a.fill(0)
+        self._prepared = (structs, tallies)
+        return self._prepared
+
+    def run_oracle(self, n, seed, threads=0, slots=None, prepared=None):
+        if prepared is None:
+            prepared = self.prepare_oracle()
+        (ph, solids, src), tallies = prepared
         st = O.Stats()
         if slots is None:
             O.lib().ggo_run(C.byref(ph), solids, len(solids), C.byref(src), n, seed, threads, C.byref(st))
