@@ -54,25 +54,42 @@ def cpu_baseline(workdir, n_per_step, seed, target_s, threads):
     sc.set_source((-600.0, 0.0, 0.0), math.degrees(math.atan(128 * math.sqrt(2.0) / 1000.0)), e, intensities=w)
     sc.build()
     prepared = sc.prepare_oracle()  # tallies allocated and faulted in outside the timed region
-    pilot = 2048
-    t0 = time.perf_counter()
-    r = sc.run_oracle(n_per_step, seed, threads=threads, slots=(0, pilot), prepared=prepared)
-    dt = time.perf_counter() - t0
-    rate = r["stats"]["primaries"] / dt
-    per_slot = r["stats"]["primaries"] / pilot
-    slots = int(min(max(pilot, target_s * rate / per_slot), 1048576 - pilot))
-    t0 = time.perf_counter()
-    r = sc.run_oracle(n_per_step, seed, threads=threads, slots=(pilot, pilot + slots), prepared=prepared)
-    dt = time.perf_counter() - t0
+    # consecutive chunks of particle slots until about target_s seconds of CPU work
+    begin, chunk, done, elapsed = 0, 2048, 0, 0.0
+    while elapsed < target_s and begin + chunk <= 1048576:
+        t0 = time.perf_counter()
+        r = sc.run_oracle(n_per_step, seed, threads=threads, slots=(begin, begin + chunk), prepared=prepared)
+        elapsed += time.perf_counter() - t0
+        done += r["stats"]["primaries"]
+        begin += chunk
+        rate = done / elapsed
+        per_slot = r["stats"]["primaries"] / chunk
+        want = (target_s - elapsed) * rate / per_slot
+        chunk = int(min(max(2048, want / 2 if want > 4096 else want + 1), 1048576 - begin))
+        if chunk <= 0:
+            break
     return {
-        "value": r["stats"]["primaries"] / dt,
+        "value": done / elapsed,
         "unit": "photons/s",
         "cores": threads,
         "kind": "port",
-        "sample": "CPU oracle (oracle/, C restatement of the reference kernels), slots %d..%d of a %d-primary "
-                  "voxel-CT step: %d primaries in %.1f s" % (pilot, pilot + slots, n_per_step,
-                                                             r["stats"]["primaries"], dt),
+        "sample": "CPU oracle (oracle/, C restatement of the reference kernels), particle slots 0..%d of a "
+                  "%d-primary voxel-CT step: %d primaries in %.1f s" % (begin, n_per_step, done, elapsed),
     }
+
+
+def measured_traffic(photons_per_step):
+    """HBM bytes per launch from the committed PMC summary of this engine source, else None."""
+    path = os.path.join(REPO, "profiles", "transport_kernel_voxel_ct.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+        from ggems_amd.profiling import source_digest
+        if rec.get("source_sha256") != source_digest(REPO) or not rec.get("traffic_bytes_per_primary"):
+            return None
+        return rec["traffic_bytes_per_primary"] * photons_per_step
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def main():
@@ -86,6 +103,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="also run a plain device copy of this size (calibrates profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; every box here has one MI355X "
@@ -97,6 +116,13 @@ def main():
     from ggems_amd import configs
     from ggems_amd.ggems import GGEMS
 
+    if args.copy_calibration_mb:
+        n = args.copy_calibration_mb * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     keep = configs.voxel_ct(args.workdir, n_particles=args.photons_per_step, seed=20240, dosimetry=True)
     g = GGEMS()
     g.initialize(args.seed)
@@ -119,6 +145,7 @@ def main():
     bpp, s, i, d = algorithmic_bytes_per_primary(stats)
     avg_kernel_s = sum(kernel_ms) / len(kernel_ms) * 1e-3
     achieved = bpp * args.photons_per_step / avg_kernel_s / 1e9
+    traffic = measured_traffic(args.photons_per_step)
     out = {
         "metric": METRIC,
         "value": value,
@@ -147,7 +174,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
             "kernel": "transport_kernel",
             "kernel_ms": avg_kernel_s * 1e3,
             "bytes_per_primary": bpp,
