@@ -132,14 +132,18 @@ def write_spectrum(path, energies_mev, intensities):
     return path
 
 
-def aluminium_mu(table_bins=220, emin_kev=1.0, emax_mev=1.0):
+MATERIALS_TXT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "_gen", "materials.txt")
+
+
+def aluminium_mu(table_bins=220, emin_kev=1.0, emax_mev=1.0, materials_txt=MATERIALS_TXT):
     """mu(E) of aluminium (cm^-1) from the transport's own cross-section tables
     (Compton + photoelectric + Rayleigh, sigma/rho interpolated, times density).
 
     GGEMSAttenuations is not used: its lookup searches the concatenated
     energy lists of all lighter elements (src/physics/GGEMSAttenuations.cc:201-212),
     which that API reproduces as is but which is no filtration model."""
-    from .ggems import GGEMSCrossSections, GGEMSMaterials, GGEMSProcessesManager
+    from .ggems import GGEMSCrossSections, GGEMSMaterials, GGEMSMaterialsDatabaseManager, GGEMSProcessesManager
+    GGEMSMaterialsDatabaseManager().set_materials(materials_txt)  # no-op when already loaded
     pm = GGEMSProcessesManager()
     pm.set_cross_section_table_number_of_bins(table_bins)
     pm.set_cross_section_table_energy_min(emin_kev, "keV")

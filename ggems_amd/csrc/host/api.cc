@@ -455,7 +455,13 @@ void attach_to_navigator_dosimetry_calculator(GGEMSDosimetryCalculator* d, char 
 }
 
 // ---- additions ---------------------------------------------------------------------------------------
-char const* get_last_error_ggems(void) { return last_error().c_str(); }
+char const* get_last_error_ggems(void) {
+  // read-and-clear, so that calls which cannot fail never leave a stale message behind
+  thread_local std::string reported;
+  reported = last_error();
+  clear_error();
+  return reported.c_str();
+}
 
 // GGEMS's clean_safely (python_module/__init__.py:121-124) tears the singletons down at exit;
 // this also deletes every registered navigator and source so a process can set up a new scene

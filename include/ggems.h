@@ -209,7 +209,8 @@ void dose_tle_navigator(GGEMSDosimetryCalculator* dose_calculator, bool is_activ
 void attach_to_navigator_dosimetry_calculator(GGEMSDosimetryCalculator* dose_calculator, char const* navigator);
 
 /* ---- additions with no counterpart in GGEMS (tests, benchmarks, tools) ----- */
-/* message of the last failed call on this thread ("" when it succeeded) */
+/* message of the most recent failed call on this thread not yet read ("" if none);
+ * reading it clears it, so call it after each call whose success matters */
 char const* get_last_error_ggems(void);
 /* deletes every registered navigator and source and restores the process-table defaults */
 void clean_safely_ggems(void);

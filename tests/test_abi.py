@@ -1,0 +1,63 @@
+"""The C-ABI libraries load on a machine without a GPU and export every
+function their headers declare (no compute call is made)."""
+
+import ctypes as C
+import os
+import re
+
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIBDIR = os.path.join(REPO, "ggems_amd", "lib")
+HEADERS = {"ggems.h": "libggems.so", "ggems_engine.h": "libggems_engine.so"}
+
+
+def declared_functions(header):
+    text = open(os.path.join(REPO, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", " ", text)
+    text = re.sub(r"#[^\n]*", " ", text)
+    text = re.sub(r"typedef\s+struct\s*\w*\s*\{.*?\}\s*\w+\s*;", " ", text, flags=re.S)
+    text = re.sub(r"\bstruct\s+\w+\s*\{.*?\}\s*;", " ", text, flags=re.S)
+    names = re.findall(r"\b([A-Za-z_]\w*)\s*\([^;{}()]*(?:\([^()]*\)[^;{}()]*)*\)\s*;", text)
+    return sorted(set(n for n in names if n not in ("if", "while", "for", "return", "sizeof")))
+
+
+@pytest.mark.parametrize("header", sorted(HEADERS))
+def test_header_symbols_exported(header):
+    names = declared_functions(header)
+    assert len(names) > 10
+    lib = C.CDLL(os.path.join(LIBDIR, HEADERS[header]))
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+# GGEMS headers whose exports are not part of this library, and why
+NOT_EXPORTED = {
+    "GGEMS/graphics/GGEMSOpenGLManager.hh": "OpenGL display, out of scope",
+    "GGEMS/navigators/GGEMSWorld.hh": "world-grid photon tracking, outside the transport hot path",
+    "GGEMS/sources/GGEMSVoxelizedSource.hh": "voxelized (emission) source, outside the transport hot path",
+    "GGEMS/navigators/GGEMSMeshedPhantom.hh": "mesh navigation, not built yet",
+    "GGEMS/geometries/GGEMSVolumeCreatorManager.hh": "volume creator, not built yet",
+    "GGEMS/geometries/GGEMSBox.hh": "volume creator, not built yet",
+    "GGEMS/geometries/GGEMSTube.hh": "volume creator, not built yet",
+    "GGEMS/geometries/GGEMSSphere.hh": "volume creator, not built yet",
+}
+
+
+def test_ggems_header_covers_the_reference_exports():
+    """Every extern "C" function of the GGEMS headers (names listed in
+    tests/golden/reference_exports.txt) is declared in include/ggems.h and
+    exported, except the headers listed in NOT_EXPORTED."""
+    ours = set(declared_functions("ggems.h"))
+    rows = [l.split() for l in open(os.path.join(REPO, "tests", "golden", "reference_exports.txt")) if l.strip()]
+    assert len(rows) == 177
+    missing = [(h, n) for h, n in rows if h not in NOT_EXPORTED and n not in ours]
+    assert not missing, missing
+    assert sum(1 for h, n in rows if h not in NOT_EXPORTED) >= 109
+
+
+def test_engine_reports_errors_without_a_device():
+    lib = C.CDLL(os.path.join(LIBDIR, "libggems_engine.so"))
+    lib.ggh_last_error.restype = C.c_char_p
+    assert isinstance(lib.ggh_last_error(), bytes)
