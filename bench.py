@@ -103,6 +103,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--hot-threshold", type=int, default=0, help="lanes a wave keeps stepping voxels for (0: engine default)")
+    ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--general-kernel", action="store_true", help="run the general kernel variant")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run a plain device copy of this size (calibrates profiler byte counters)")
     args = ap.parse_args()
@@ -127,6 +130,7 @@ def main():
     g = GGEMS()
     g.initialize(args.seed)
     eng = g.engine()
+    eng.set_launch(args.hot_threshold, args.blocks_per_cu, args.general_kernel)
     for _ in range(args.warmup):
         eng.run(args.photons_per_step)
     eng.reset_tallies()
@@ -176,6 +180,8 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "kernel": "transport_kernel",
+            "kernel_variant": "general" if eng.last_grid()[2] else "specialised",
+            "grid": list(eng.last_grid()[:2]),
             "kernel_ms": avg_kernel_s * 1e3,
             "bytes_per_primary": bpp,
         },

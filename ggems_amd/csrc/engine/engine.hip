@@ -1,24 +1,34 @@
 // engine.hip -- MI355X photon-transport engine behind the thin C ABI of
 // include/ggems_engine.h.
 //
-// One launch transports a whole run.  Thread s owns particle slot s: its
-// JKISS32 state stays in registers and it transports, in order, photon s of
-// every batch (src/global/GGEMS.cc:303-363 runs batches of
-// MAXIMUM_PARTICLES slots; slot s of every batch continues stream s).  Each
+// One persistent launch transports a whole run.  A particle slot is the
+// unit of work: slot s owns JKISS32 stream s and transports, in order,
+// photon s of every batch (src/global/GGEMS.cc:303-363 runs batches of
+// MAXIMUM_PARTICLES slots and slot s of every batch continues stream s).
+// A lane takes a slot from a device-wide queue, keeps its stream in
+// registers while it transports the slot's photons, writes the stream back
+// and takes the next slot, so lanes stay busy until the queue drains.  Each
 // photon walks the reference's hop loop (find solid -> project -> track)
-// inside the thread, so the host never synchronises per hop.
+// inside the lane; the host never synchronises per hop.
 //
-// Divergence: the voxel step is the hot path.  A wave alternates a cold
-// phase (every lane not inside a voxel solid advances: new primary, hop
-// bookkeeping, detector transport) with a hot phase that steps voxels while
-// enough lanes are inside a phantom; lanes that leave the phantom wait for
-// the next cold phase instead of forcing the whole wave through the cold
-// code every iteration.
+// Divergence.  The voxel step is the hot path (hundreds of steps per
+// primary, about one interaction per hundred steps).  A lane's work is a
+// small state machine; a wave alternates
+//   * a hot phase: voxel steps while enough lanes are inside a phantom.  A
+//     step that ends in an interaction, an exit or the energy cut only
+//     records why the lane stopped, so the step itself stays short and
+//     branch-light;
+//   * a cold phase: every stopped lane resolves its interaction, leaves the
+//     phantom, starts its next primary, crosses to the next solid or tracks
+//     through a detector module, until it is back inside a phantom.
+// The state machine only reorders work between lanes; each lane consumes
+// its stream in exactly the reference order.
 #include <hip/hip_runtime.h>
 
 #include <fcntl.h>
 #include <unistd.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -33,22 +43,32 @@ using namespace ggh;
 
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 512;            // 8 waves; 2 blocks per CU at 4 waves per SIMD
+constexpr int kWavesPerEU = 4;         // register budget: 128 VGPRs per lane
+constexpr int kLdsBudget = 64 * 1024;  // step tables + energy bins staged per block
 
 // ---------------------------------------------------------------------------
 // device-side scene description
 // ---------------------------------------------------------------------------
 enum SolidType : int32_t { kVoxel = 0, kBoxSolid = 1 };
 
+// Everything the voxel step reads about its phantom.  rvs = RN(1/voxel_size)
+// (exact division, see div_rn); in_lo/in_hi = border -/+ the geometry
+// tolerance, the bounds IsParticleInAABB compares against.
+struct VoxGeom {
+  f3 lo, vs, rvs, in_lo, in_hi;
+  int32_t nx, ny, nz, nxy;
+  const uint8_t* labels;
+  int32_t mat_offset;
+  int32_t solid;
+};
+
 struct DevSolid {
   Mat34 m;           // local->global rows 0..2
   Box border;        // local borders
   int32_t type;
   int32_t mat_offset;
-  // voxel solid
-  f3 voxel_size;
-  int32_t nv[3];
-  const uint8_t* labels;
+  VoxGeom vox;       // voxel solid
   // box solid (CT module)
   int32_t ne[3];
   f3 box_size;
@@ -64,9 +84,11 @@ struct RunParams {
   int32_t nbins;
   int32_t n_mat;
   int32_t n_proc;
-  int32_t proc[3];
+  int32_t proc[3];            // activation order
   const float* ebins;
-  const float4* cs4;          // [n_mat*nbins] {sigma_C, sigma_PE, sigma_R, photon_cut}
+  const float4* tab_sig;      // [n_mat*nbins] {sigma of proc[0], proc[1], proc[2], photon cut}
+  const float4* tab_rcp;      // [n_mat*nbins] {RN(1/sigma) of proc[0..2], 0}
+  const float* cs_r;          // [n_mat*nbins] Rayleigh sigma of the material (element selection)
   const float* cs_atom_r;     // [101*nbins]
   const int32_t* n_elem;
   const int32_t* elem_idx;
@@ -94,51 +116,123 @@ struct RunParams {
   uint32_t* rng;              // SoA [5][rng_stride]
   uint32_t rng_stride;
   uint32_t n_slots;
-  uint64_t n_batches, base, rem;
+  uint32_t n_batches, base, rem;
+  uint32_t* queue;            // next slot to hand out
   unsigned long long* stats;  // ggh_stats fields
   int32_t hot_threshold;
+  VoxGeom vox0;               // the only voxel solid, when there is exactly one
 };
 
-struct Photon {
-  f3 p, d;
+// tables as seen by a block: LDS copies or the global arrays
+struct Tabs {
+  const float4* sig;
+  const float4* rcp;
+  const float* ebins;
+};
+
+// ---------------------------------------------------------------------------
+// lane state
+// ---------------------------------------------------------------------------
+enum Mode : int32_t {
+  kInVoxel = 0,    // stepping through a voxel solid (hot)
+  kInteract = 1,   // a voxel step ended on a discrete process (pend = proc | gmat << 2)
+  kCut = 2,        // a voxel step left the photon at or below the material's energy cut
+  kExit = 3,       // left the voxel solid
+  kNeedPrimary = 4,
+  kHop = 5,
+  kNeedSlot = 6,
+  kDone = 7
+};
+
+// p and ray hold the local frame of the solid while mode is kInVoxel ..
+// kExit, the global frame otherwise.
+struct Lane {
+  Rng r;
+  f3 p;
+  Ray ray;
   float e;
   int32_t eid;
-  int32_t status;
-  int32_t scatter;
+  int32_t mode;
   int32_t solid;
-  float dist;
+  int32_t hop;
+  int32_t scatter;
+  int32_t pend;
+  uint32_t todo;
+  uint32_t slot;
 };
 
 struct Counters {
-  uint32_t primaries, voxel_steps, interactions, detector_steps, detector_interactions, hops;
+  uint32_t primaries, interactions, detector_steps, detector_interactions, hops;
 };
 
 // ---------------------------------------------------------------------------
 // physics (include/GGEMS/navigators/GGEMSPhotonNavigator.hh:57-131)
 // ---------------------------------------------------------------------------
-__device__ inline float sigma_of(const float4& s, int pid) { return pid == 0 ? s.x : (pid == 1 ? s.y : s.z); }
+// -logf(u) for the uniform u of sum s, with the arithmetic of the device
+// logf for normal arguments (log2 by v_log_f32, then times ln 2 in two
+// pieces).  u is 0 or at least 2^-32, never subnormal, so logf's subnormal
+// rescaling is left out.  For s == 0 (u = 0, where the reference's
+// -log(0)/sigma is +inf) the result is NaN, which every caller compares
+// with `<` exactly as +inf.
+__device__ inline float neg_log_uniform(uint32_t s) {
+  const float x = (float)s * 0x1p-32f;
+  uint32_t b = __float_as_uint(x);
+  b -= ((b & 0x7fffffu) - 1u) >= 4218928u ? 2u : 1u;  // uniform_from_sum; s == 0 leaves a NaN pattern
+  const float r = __builtin_amdgcn_logf(__uint_as_float(b));
+  const float c = __uint_as_float(0x3f317217u), cc = __uint_as_float(0x3377d1cfu);
+  const float y = r * c;
+  float t = __builtin_fmaf(r, c, -y);
+  t = __builtin_fmaf(cc, r, t);
+  return -(y + t);
+}
 
 // GetPhotonNextInteraction: competing exponentials over the activated
-// processes with the left-bin cross section (GGEMSPhotonNavigator.hh:57-93)
-__device__ inline float next_interaction(const RunParams& P, const float4& sig, Rng& r, int& proc) {
+// processes, in activation order, with the left-bin cross section
+// (GGEMSPhotonNavigator.hh:57-93).  The table entry holds the cross sections
+// in activation order and their correctly rounded reciprocals, so that
+// -log(u)/sigma is the IEEE quotient without a division.  NP is the number
+// of activated processes, or -1 to read it from the run parameters.
+template <int NP>
+__device__ inline float next_interaction(const RunParams& P, const float4& s, const float4& y, const float* lg,
+                                         int& proc) {
   float next = kOutOfWorld;
   proc = kTransportation;  // NO_PROCESS == TRANSPORTATION == 99
-  for (int i = 0; i < P.n_proc; ++i) {
-    int pid = P.proc[i];
-    float d = -logf(r.uniform()) / sigma_of(sig, pid);
-    if (d < next) {
-      next = d;
-      proc = pid;
-    }
+  const int np = NP >= 0 ? NP : P.n_proc;
+  if (np > 0) {
+    const float d = div_rn(lg[0], s.x, y.x);
+    const bool w = d < next;
+    next = w ? d : next;
+    proc = w ? P.proc[0] : proc;
+  }
+  if (np > 1) {
+    const float d = div_rn(lg[1], s.y, y.y);
+    const bool w = d < next;
+    next = w ? d : next;
+    proc = w ? P.proc[1] : proc;
+  }
+  if (np > 2) {
+    const float d = div_rn(lg[2], s.z, y.z);
+    const bool w = d < next;
+    next = w ? d : next;
+    proc = w ? P.proc[2] : proc;
   }
   return next;
 }
 
+// the -log(u) of every activated process, drawn in activation order
+template <int NP>
+__device__ inline void draw_logs(const RunParams& P, Rng& r, float* lg) {
+  const int np = NP >= 0 ? NP : P.n_proc;
+  if (np > 0) lg[0] = neg_log_uniform(r.next_sum());
+  if (np > 1) lg[1] = neg_log_uniform(r.next_sum());
+  if (np > 2) lg[2] = neg_log_uniform(r.next_sum());
+}
+
 // KleinNishinaComptonSampleSecondaries (GGEMSComptonScatteringModels.hh:47-132)
-__device__ void compton(Photon& g, Rng& r) {
-  float kE0 = g.e;
+__device__ void compton(float& e, f3& dir, Rng& r) {
+  float kE0 = e;
   float kE0_MeC2 = kE0 / kElectronMassC2;
-  f3 dir0 = g.d;
+  f3 dir0 = dir;
   float kEps0 = 1.0f / (1.0f + 2.0f * kE0_MeC2);
   float kEps0Eps0 = kEps0 * kEps0;
   float kAlpha1 = -logf(kEps0);
@@ -168,27 +262,23 @@ __device__ void compton(Photon& g, Rng& r) {
   float sintheta = sqrtf(sint2);
   float phi = r.uniform() * kTwoPi;
   f3 gd = mk(sintheta * cosf(phi), sintheta * sinf(phi), costheta);
-  gd = normalize(rotate_unit_z(gd, dir0));
-  g.e = kE0 * epsilon;
-  g.d = gd;
+  dir = normalize(rotate_unit_z(gd, dir0));
+  e = kE0 * epsilon;
 }
 
-// LivermoreRayleighSampleSecondaries (GGEMSRayleighScatteringModels.hh:169-316)
-__device__ void rayleigh(const RunParams& P, Photon& g, int gmat, Rng& r) {
-  float kE0 = g.e;
-  if (kE0 <= 250.0e-6f) {
-    g.status = kDead;
-    return;
-  }
-  f3 dir0 = g.d;
+// LivermoreRayleighSampleSecondaries (GGEMSRayleighScatteringModels.hh:169-316).
+// Returns false when the photon is killed (E <= 250 eV).
+__device__ bool rayleigh(const RunParams& P, const float* ebins, float kE0, int eid, f3& dir, int gmat,
+                                      Rng& r) {
+  if (kE0 <= 250.0e-6f) return false;
+  f3 dir0 = dir;
   const int nb = P.nbins;
   const int nm1 = P.n_elem[gmat] - 1;
   const int mix = P.elem_idx[gmat];
-  const int eid = g.eid;
   int selz = P.z[mix + nm1];
   if (nm1 > 0) {
-    float e_lo = P.ebins[eid], e_hi = P.ebins[eid + 1];
-    float kCS = lin_interp(e_lo, P.cs4[gmat * nb + eid].z, e_hi, P.cs4[gmat * nb + eid + 1].z, kE0);
+    float e_lo = ebins[eid], e_hi = ebins[eid + 1];
+    float kCS = lin_interp(e_lo, P.cs_r[gmat * nb + eid], e_hi, P.cs_r[gmat * nb + eid + 1], kE0);
     float x = r.uniform() * kCS;
     float cross = 0.0f;
     for (int i = 0; i < nm1; ++i) {
@@ -240,19 +330,24 @@ __device__ void rayleigh(const RunParams& P, Photon& g, int gmat, Rng& r) {
   float phi = kTwoPi * r.uniform();
   float sintheta = sqrtf((1.0f - costheta) * (1.0f + costheta));
   f3 gd = mk(sintheta * cosf(phi), sintheta * sinf(phi), costheta);
-  g.d = normalize(rotate_unit_z(gd, dir0));
+  dir = normalize(rotate_unit_z(gd, dir0));
+  return true;
 }
 
-// PhotonDiscreteProcess (GGEMSPhotonNavigator.hh:109-131)
-__device__ inline void discrete(const RunParams& P, Photon& g, int proc, int gmat, Rng& r) {
+// PhotonDiscreteProcess (GGEMSPhotonNavigator.hh:109-131).  Returns false
+// when the photon dies.
+__device__ inline bool discrete(const RunParams& P, const float* ebins, int proc, int gmat, float& e, int eid, f3& dir,
+                                Rng& r) {
   if (proc == kCompton) {
-    compton(g, r);
-  } else if (proc == kPhotoelectric) {
-    g.status = kDead;
-    g.e = 0.0f;
-  } else if (proc == kRayleigh) {
-    rayleigh(P, g, gmat, r);
+    compton(e, dir, r);
+    return true;
   }
+  if (proc == kPhotoelectric) {
+    e = 0.0f;
+    return false;
+  }
+  if (proc == kRayleigh) return rayleigh(P, ebins, e, eid, dir, gmat, r);
+  return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -280,124 +375,108 @@ __device__ inline void dose_record(const RunParams& P, float edep, f3 p) {
 }
 
 // ---------------------------------------------------------------------------
-// voxel solid transport (src/kernels/TrackThroughGGEMSVoxelizedSolid.cl:76-323)
+// voxel solid transport (src/kernels/TrackThroughGGEMSVoxelizedSolid.cl:133-310)
 // ---------------------------------------------------------------------------
-struct VoxelWalk {
-  f3 lp;          // local position
-  Ray ld;         // local direction + reciprocals
-  Box border;
-  f3 vs;
-  int32_t nx, ny, nz;
-  const uint8_t* labels;
-  int32_t mat_offset;
-  int32_t dose;   // this solid carries the dose grid
-  int32_t solid;  // index of the solid being walked
-};
-
-__device__ inline void voxel_enter(VoxelWalk& v, const Photon& g, const DevSolid& s, int solid, int dose) {
-  v.solid = solid;
-  v.lp = g2l_pos(s.m, g.p);
-  v.ld.set(g2l_dir(s.m, g.d));
-  v.border = s.border;
-  v.vs = s.voxel_size;
-  v.nx = s.nv[0];
-  v.ny = s.nv[1];
-  v.nz = s.nv[2];
-  v.labels = s.labels;
-  v.mat_offset = s.mat_offset;
-  v.dose = dose;
+__device__ inline bool in_bounds(f3 p, const VoxGeom& G) {
+  // IsParticleInAABB with its tolerance folded into the bounds
+  return !((p.x < G.in_lo.x) | (p.x > G.in_hi.x) | (p.y < G.in_lo.y) | (p.y > G.in_hi.y) | (p.z < G.in_lo.z) |
+           (p.z > G.in_hi.z));
 }
 
-// One iteration of the reference do-while.  Returns true while the photon
-// stays inside the solid and alive.  kWalk: no interaction (sigma == 0),
-// records the visited voxel index instead.
-template <bool kWalk>
-__device__ inline bool voxel_step(const RunParams& P, const float4* __restrict__ cs, VoxelWalk& v, Photon& g, Rng& r,
-                                  Counters& c, int32_t* walk_out, int& walk_n, int walk_max) {
-  int vx = (int)((v.lp.x - v.border.lo.x) / v.vs.x);
-  int vy = (int)((v.lp.y - v.border.lo.y) / v.vs.y);
-  int vz = (int)((v.lp.z - v.border.lo.z) / v.vs.z);
-  // The reference checks the upper bound only; a negative index would read
-  // outside the label volume, so it ends the walk like an exit.
-  if (vx >= v.nx || vy >= v.ny || vz >= v.nz || vx < 0 || vy < 0 || vz < 0) {
-    g.dist = kOutOfWorld;
-    g.solid = -1;
+// Index of the voxel holding p: (int)((p - border_min) / voxel_size) with the
+// IEEE quotient.  False when outside the label volume (the reference checks
+// the upper bound only; a negative index would read outside the volume, so
+// it ends the walk like an exit).
+__device__ inline bool voxel_of(f3 p, const VoxGeom& G, int& vx, int& vy, int& vz) {
+  vx = (int)div_rn(p.x - G.lo.x, G.vs.x, G.rvs.x);
+  vy = (int)div_rn(p.y - G.lo.y, G.vs.y, G.rvs.y);
+  vz = (int)div_rn(p.z - G.lo.z, G.vs.z, G.rvs.z);
+  return ((uint32_t)vx < (uint32_t)G.nx) & ((uint32_t)vy < (uint32_t)G.ny) & ((uint32_t)vz < (uint32_t)G.nz);
+}
+
+__device__ inline Box voxel_box(const VoxGeom& G, int vx, int vy, int vz) {
+  Box b;
+  b.lo = mk(G.lo.x + (float)vx * G.vs.x, G.lo.y + (float)vy * G.vs.y, G.lo.z + (float)vz * G.vs.z);
+  b.hi = add(b.lo, G.vs);
+  return b;
+}
+
+// One iteration of the reference do-while, up to the point where the step's
+// outcome is known.  Returns true when the step counts as a voxel step (the
+// voxel index was inside the volume).  The label gather is issued first and
+// the random draws of the step, which do not depend on it, cover its latency.
+template <int NP, bool kTrack>
+__device__ inline bool hot_step(const RunParams& P, const Tabs& T, const VoxGeom& G, Lane& L) {
+  int vx, vy, vz;
+  if (!voxel_of(L.p, G, vx, vy, vz)) {
+    L.mode = kExit;
     return false;
   }
-  int vid = vx + vy * v.nx + vz * v.nx * v.ny;
-  int proc = kTransportation;
-  float next = kOutOfWorld;
-  int gmat = 0;
-  float4 sig = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (kWalk) {
-    if (walk_n < walk_max) walk_out[walk_n] = vid;
-    ++walk_n;
-  } else {
-    gmat = v.mat_offset + v.labels[vid];
-    sig = cs[gmat * P.nbins + g.eid];
-    next = next_interaction(P, sig, r, proc);
-    ++c.voxel_steps;
-  }
-  Box vox;
-  vox.lo = mk(v.border.lo.x + (float)vx * v.vs.x, v.border.lo.y + (float)vy * v.vs.y,
-              v.border.lo.z + (float)vz * v.vs.z);
-  vox.hi = add(vox.lo, v.vs);
-  safety_inside(v.lp, vox);
-  float db = distance_to_aabb(v.lp, v.ld, vox);
+  const uint32_t label = G.labels[vx + vy * G.nx + vz * G.nxy];
+  float lg[3];
+  draw_logs<NP>(P, L.r, lg);
+  const Box vox = voxel_box(G, vx, vy, vz);
+  safety_inside(L.p, vox);
+  const float db = distance_to_aabb(L.p, L.ray, vox);
+  const int gmat = G.mat_offset + (int)label;
+  const int ti = gmat * P.nbins + L.eid;
+  const float4 sig = T.sig[ti];
+  const float4 rcp = T.rcp[ti];
+  int proc;
+  float next = next_interaction<NP>(P, sig, rcp, lg, proc);
   if (db <= next) {
     next = db + kTol;
     proc = kTransportation;
-    if (!kWalk && v.dose && P.ptrack) {
-      int id = dosel_index(P, v.lp);
+    if (kTrack && P.ptrack && G.solid == P.dose_solid) {
+      int id = dosel_index(P, L.p);
       if (id >= 0) atomicAdd(&P.ptrack[id], 1);
     }
   }
-  v.lp = add(v.lp, scl(v.ld.d, next));
-  safety_outside(v.lp, vox);
-  if (!in_aabb(v.lp, v.border)) {
-    g.dist = kOutOfWorld;
-    g.solid = -1;
-    return false;
-  }
-  if (kWalk) return true;
-  float e0 = g.e;
-  if (proc != kTransportation) {
-    ++c.interactions;
-    g.d = v.ld.d;
-    discrete(P, g, proc, gmat, r);
-    if (proc == kCompton || proc == kRayleigh) g.scatter = 1;
-    if (v.dose) dose_record(P, e0 - g.e, v.lp);
-    if (proc != kPhotoelectric) {
-      v.ld.set(g.d);
-      if (g.e != e0) g.eid = binary_search(g.e, P.ebins, P.nbins);
-    }
-  }
-  if (g.e <= sig.w) {  // photon_energy_cut_[material]
-    if (v.dose) dose_record(P, g.e, v.lp);
-    g.status = kDead;
-  }
-  return g.status == kAlive;
+  L.p = add(L.p, scl(L.ray.d, next));
+  safety_outside(L.p, vox);
+  // outcome by selects: stores to different lane fields on different paths
+  // would be merged by the compiler into one store through a selected
+  // address, which keeps the lane state out of registers
+  const bool out = !in_bounds(L.p, G);
+  const bool inter = !out & (proc != kTransportation);
+  const bool cut = !out & !inter & (L.e <= sig.w);  // photon_energy_cut_[material]
+  L.mode = out ? kExit : (inter ? kInteract : (cut ? kCut : kInVoxel));
+  L.pend = inter ? (proc | (gmat << 2)) : L.pend;
+  return true;
 }
 
-__device__ inline void voxel_leave(const VoxelWalk& v, Photon& g, const DevSolid& s) {
-  g.p = l2g_pos(s.m, v.lp);
-  g.d = l2g_dir(s.m, v.ld.d);
+// The same walk with no interaction (sigma == 0): records the voxel indices.
+__device__ inline bool walk_step(const VoxGeom& G, f3& p, const Ray& ray, int32_t* out, int& n, int max_out) {
+  int vx, vy, vz;
+  if (!voxel_of(p, G, vx, vy, vz)) return false;
+  if (n < max_out) out[n] = vx + vy * G.nx + vz * G.nxy;
+  ++n;
+  const Box vox = voxel_box(G, vx, vy, vz);
+  safety_inside(p, vox);
+  const float db = distance_to_aabb(p, ray, vox);
+  float next = kOutOfWorld;
+  if (db <= next) next = db + kTol;
+  p = add(p, scl(ray.d, next));
+  safety_outside(p, vox);
+  return in_bounds(p, G);
 }
 
 // ---------------------------------------------------------------------------
 // CT module transport (src/kernels/TrackThroughGGEMSSolidBox.cl:56-261)
 // ---------------------------------------------------------------------------
-__device__ void track_box(const RunParams& P, const float4* __restrict__ cs, const DevSolid& s, Photon& g, Rng& r,
-                          Counters& c) {
-  f3 lp = g2l_pos(s.m, g.p);
+__device__ void track_box(const RunParams& P, const Tabs& T, const DevSolid& s, Lane& L, Counters& c) {
+  f3 lp = g2l_pos(s.m, L.p);
   Ray ld;
-  ld.set(g2l_dir(s.m, g.d));
+  ld.set(g2l_dir(s.m, L.ray.d));
   const Box b = s.border;
   const int gmat = s.mat_offset;
+  bool alive = true;
   do {
     int proc;
-    float4 sig = cs[gmat * P.nbins + g.eid];
-    float next = next_interaction(P, sig, r, proc);
+    const int ti = gmat * P.nbins + L.eid;
+    float lg[3];
+    draw_logs<-1>(P, L.r, lg);
+    float next = next_interaction<-1>(P, T.sig[ti], T.rcp[ti], lg, proc);
     ++c.detector_steps;
     safety_inside(lp, b);
     float db = distance_to_aabb(lp, ld, b);
@@ -407,20 +486,16 @@ __device__ void track_box(const RunParams& P, const float4* __restrict__ cs, con
     }
     lp = add(lp, scl(ld.d, next));
     safety_outside(lp, b);
-    if (!in_aabb(lp, b)) {
-      g.dist = kOutOfWorld;
-      g.solid = -1;
-      break;
-    }
-    if (g.e < s.threshold) g.status = kDead;
+    if (!in_aabb(lp, b)) break;  // leaves the module: back to navigation
+    if (L.e < s.threshold) alive = false;
     if (proc != kTransportation) {
       ++c.detector_interactions;
-      float e0 = g.e;
-      g.d = ld.d;
-      discrete(P, g, proc, gmat, r);
+      float e0 = L.e;
+      f3 dir = ld.d;
+      if (!discrete(P, T.ebins, proc, gmat, L.e, L.eid, dir, L.r)) alive = false;
       if (proc != kPhotoelectric) {
-        ld.set(g.d);
-        if (g.e != e0) g.eid = binary_search(g.e, P.ebins, P.nbins);
+        ld.set(dir);
+        if (L.e != e0) L.eid = binary_search(L.e, T.ebins, P.nbins);
       }
       if (proc == kPhotoelectric || proc == kCompton) {
         float esx = s.box_size.x / (float)s.ne[0];
@@ -430,19 +505,21 @@ __device__ void track_box(const RunParams& P, const float4* __restrict__ cs, con
         if (ix >= 0 && ix < s.ne[0] && iy >= 0 && iy < s.ne[1]) {
           int id = ix + iy * s.ne[0];
           atomicAdd(&s.histogram[id], 1);
-          if (s.scatter && g.scatter) atomicAdd(&s.scatter[id], 1);
+          if (s.scatter && L.scatter) atomicAdd(&s.scatter[id], 1);
         }
       }
     }
-  } while (g.status == kAlive);
-  g.p = l2g_pos(s.m, lp);
-  g.d = l2g_dir(s.m, ld.d);
+  } while (alive);
+  L.p = l2g_pos(s.m, lp);
+  L.ray.d = l2g_dir(s.m, ld.d);
+  L.mode = alive ? kHop : kNeedPrimary;
 }
 
 // ---------------------------------------------------------------------------
 // source (src/kernels/GetPrimariesGGEMSXRaySource.cl:52-143)
 // ---------------------------------------------------------------------------
-__device__ void sample_primary(const RunParams& P, Rng& r, Photon& g) {
+__device__ void sample_primary(const RunParams& P, const float* ebins, Rng& r, f3& pos, f3& dir,
+                                            float& energy, int& eid) {
   double phi = r.uniform();
   double theta = r.uniform();
   phi *= (double)kTwoPi;
@@ -463,15 +540,11 @@ __device__ void sample_primary(const RunParams& P, Rng& r, Photon& g) {
     rnd = r.uniform();
     idx = binary_search(rnd, P.cdf, P.n_energy_arg + 1);
   }
-  g.e = (!P.is_interp) ? P.spectrum[idx]
-                       : lin_interp(P.cdf[idx], P.spectrum[idx], P.cdf[idx + 1], P.spectrum[idx + 1], rnd);
-  g.p = gp;
-  g.d = d;
-  g.status = kAlive;
-  g.scatter = 0;
-  g.solid = -1;
-  g.dist = kOutOfWorld;
-  g.eid = binary_search(g.e, P.ebins, P.nbins);
+  energy = (!P.is_interp) ? P.spectrum[idx]
+                          : lin_interp(P.cdf[idx], P.spectrum[idx], P.cdf[idx + 1], P.spectrum[idx + 1], rnd);
+  pos = gp;
+  dir = d;
+  eid = binary_search(energy, ebins, P.nbins);
 }
 
 // ---------------------------------------------------------------------------
@@ -479,38 +552,34 @@ __device__ void sample_primary(const RunParams& P, Rng& r, Photon& g) {
 // ---------------------------------------------------------------------------
 // particle_solid_distance_* over every solid in registration order
 // (src/kernels/ParticleSolidDistanceGGEMSVoxelizedSolid.cl:43-108) followed by
-// project_to_* (src/kernels/ProjectToGGEMSVoxelizedSolid.cl:44-129).
-// Returns false when the photon leaves the world.
-__device__ inline bool find_and_project(const RunParams& P, Photon& g) {
+// project_to_* (src/kernels/ProjectToGGEMSVoxelizedSolid.cl:44-129).  Returns
+// the solid the photon now sits in, or -1 when it leaves the world.
+__device__ int find_and_project(const RunParams& P, f3& pos, f3 dir) {
+  float dist = kOutOfWorld;
+  int solid = -1;
   for (int k = 0; k < P.n_solids; ++k) {
-    if (g.dist == 0.0f) break;
     const DevSolid& s = P.solids[k];
-    f3 lp = g2l_pos(s.m, g.p);
+    f3 lp = g2l_pos(s.m, pos);
     if (in_aabb(lp, s.border)) {
-      g.dist = 0.0f;
-      g.solid = k;
+      dist = 0.0f;
+      solid = k;
       break;
     }
     Ray lr;
-    lr.set(g2l_dir(s.m, g.d));
+    lr.set(g2l_dir(s.m, dir));
     float d = distance_to_aabb(lp, lr, s.border);
-    if (d < g.dist) {
-      g.dist = d;
-      g.solid = k;
+    if (d < dist) {
+      dist = d;
+      solid = k;
     }
   }
-  if (g.solid == -1 || g.dist == kOutOfWorld) {
-    g.solid = -1;
-    g.status = kDead;
-    return false;
-  }
-  const DevSolid& s = P.solids[g.solid];
-  g.p = add(g.p, scl(g.d, g.dist + kTol));
-  f3 lp = g2l_pos(s.m, g.p);
+  if (solid == -1 || dist == kOutOfWorld) return -1;
+  const DevSolid& s = P.solids[solid];
+  pos = add(pos, scl(dir, dist + kTol));
+  f3 lp = g2l_pos(s.m, pos);
   safety_inside(lp, s.border);
-  g.p = l2g_pos(s.m, lp);
-  g.dist = 0.0f;
-  return true;
+  pos = l2g_pos(s.m, lp);
+  return solid;
 }
 
 __device__ inline uint32_t wave_sum(uint32_t v) {
@@ -518,106 +587,185 @@ __device__ inline uint32_t wave_sum(uint32_t v) {
   return v;
 }
 
-enum Mode : int32_t { kNeedPrimary = 0, kHop = 1, kInVoxel = 2, kDone = 3 };
+__device__ inline void load_stream(const RunParams& P, uint32_t slot, Rng& r) {
+  r.x = P.rng[0 * (size_t)P.rng_stride + slot];
+  r.y = P.rng[1 * (size_t)P.rng_stride + slot];
+  r.z = P.rng[2 * (size_t)P.rng_stride + slot];
+  r.w = P.rng[3 * (size_t)P.rng_stride + slot];
+  r.c = P.rng[4 * (size_t)P.rng_stride + slot];
+}
 
-template <bool kLDS>
-__global__ __launch_bounds__(kBlock) void transport_kernel(RunParams P) {
-  extern __shared__ float4 s_cs[];
-  const float4* cs = P.cs4;
+__device__ inline void store_stream(const RunParams& P, uint32_t slot, const Rng& r) {
+  P.rng[0 * (size_t)P.rng_stride + slot] = r.x;
+  P.rng[1 * (size_t)P.rng_stride + slot] = r.y;
+  P.rng[2 * (size_t)P.rng_stride + slot] = r.z;
+  P.rng[3 * (size_t)P.rng_stride + slot] = r.w;
+  P.rng[4 * (size_t)P.rng_stride + slot] = r.c;
+}
+
+// the cold work of one lane, one state at a time in an order that lets a
+// lane pass through several states in one call
+template <bool kSingle>
+__device__ inline void cold_step(const RunParams& P, const Tabs& T, Lane& L, VoxGeom& G, Counters& c) {
+  if (L.mode == kInteract) {
+    const int proc = L.pend & 3, gmat = L.pend >> 2;
+    ++c.interactions;
+    const float e0 = L.e;
+    f3 dir = L.ray.d;
+    bool alive = discrete(P, T.ebins, proc, gmat, L.e, L.eid, dir, L.r);
+    if (proc == kCompton || proc == kRayleigh) L.scatter = 1;
+    const bool dose = L.solid == P.dose_solid;
+    if (dose) dose_record(P, e0 - L.e, L.p);
+    if (proc != kPhotoelectric) {
+      L.ray.set(dir);
+      if (L.e != e0) L.eid = binary_search(L.e, T.ebins, P.nbins);
+    }
+    if (L.e <= T.sig[gmat * P.nbins].w) {  // photon_energy_cut_[material]
+      if (dose) dose_record(P, L.e, L.p);
+      alive = false;
+    }
+    L.mode = alive ? kInVoxel : kNeedPrimary;
+  }
+  if (L.mode == kCut) {
+    if (L.solid == P.dose_solid) dose_record(P, L.e, L.p);
+    L.mode = kNeedPrimary;
+  }
+  if (L.mode == kExit) {
+    const DevSolid& s = P.solids[L.solid];
+    L.p = l2g_pos(s.m, L.p);
+    L.ray.d = l2g_dir(s.m, L.ray.d);
+    L.mode = kHop;
+  }
+  if (L.mode == kNeedPrimary && L.todo == 0) {
+    if (L.slot != 0xffffffffu) store_stream(P, L.slot, L.r);
+    L.mode = kNeedSlot;
+  }
+  if (L.mode == kNeedSlot) {
+    // one queue update per wave: the first lane that needs a slot takes a
+    // contiguous range for all of them
+    const uint64_t need = __ballot(1);
+    const int leader = __ffsll((unsigned long long)need) - 1;
+    const int lane = __lane_id();
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(P.queue, (uint32_t)__popcll(need));
+    base = __shfl(base, leader, 64);
+    const uint32_t my = base + (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
+    if (my < P.n_slots) {
+      L.slot = my;
+      load_stream(P, my, L.r);
+      // batch b holds base + (b < rem) photons; slot s takes photon s of each
+      L.todo = my < P.base ? P.n_batches : (my == P.base ? P.rem : 0u);
+      L.mode = kNeedPrimary;
+    } else {
+      L.slot = 0xffffffffu;
+      L.mode = kDone;
+    }
+  }
+  if (L.mode == kNeedPrimary) {
+    if (L.todo == 0) {
+      L.mode = kNeedSlot;  // empty slot: take another one on the next pass
+    } else {
+      --L.todo;
+      sample_primary(P, T.ebins, L.r, L.p, L.ray.d, L.e, L.eid);
+      ++c.primaries;
+      L.scatter = 0;
+      L.hop = 0;
+      L.mode = kHop;
+    }
+  }
+  if (L.mode == kHop) {
+    if (L.hop >= GGH_MAX_HOPS) {
+      L.mode = kNeedPrimary;
+      return;
+    }
+    ++L.hop;
+    ++c.hops;
+    const int k = find_and_project(P, L.p, L.ray.d);
+    if (k < 0) {
+      L.mode = kNeedPrimary;
+      return;
+    }
+    L.solid = k;
+    const DevSolid& s = P.solids[k];
+    if (s.type == kVoxel) {
+      L.p = g2l_pos(s.m, L.p);
+      L.ray.set(g2l_dir(s.m, L.ray.d));
+      if (!kSingle) G = s.vox;
+      L.mode = kInVoxel;
+    } else {
+      track_box(P, T, s, L, c);
+    }
+  }
+}
+
+// NP: activated processes (-1: read at run time); kSingle: the scene has
+// exactly one voxel solid, whose geometry the voxel step reads from the
+// kernel arguments; kLDS: step tables staged in LDS; kTrack: photon
+// fluence tracking may be on.
+template <int NP, bool kSingle, bool kLDS, bool kTrack>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWavesPerEU)))
+void transport_kernel(RunParams P) {
+  extern __shared__ float4 s_tab[];
+  Tabs T = {P.tab_sig, P.tab_rcp, P.ebins};
   if (kLDS) {
     const int n = P.n_mat * P.nbins;
-    for (int i = threadIdx.x; i < n; i += kBlock) s_cs[i] = P.cs4[i];
+    float4* s_sig = s_tab;
+    float4* s_rcp = s_tab + n;
+    float* s_eb = reinterpret_cast<float*>(s_tab + 2 * n);
+    for (int i = threadIdx.x; i < n; i += kBlock) {
+      s_sig[i] = P.tab_sig[i];
+      s_rcp[i] = P.tab_rcp[i];
+    }
+    for (int i = threadIdx.x; i < P.nbins; i += kBlock) s_eb[i] = P.ebins[i];
     __syncthreads();
-    cs = s_cs;
+    T.sig = s_sig;
+    T.rcp = s_rcp;
+    T.ebins = s_eb;
   }
-  const uint32_t slot = blockIdx.x * kBlock + threadIdx.x;
-  const bool valid = slot < P.n_slots;
-  Rng r = {0, 0, 0, 0, 0};
-  uint64_t todo = 0;
-  if (valid) {
-    r.x = P.rng[0 * (size_t)P.rng_stride + slot];
-    r.y = P.rng[1 * (size_t)P.rng_stride + slot];
-    r.z = P.rng[2 * (size_t)P.rng_stride + slot];
-    r.w = P.rng[3 * (size_t)P.rng_stride + slot];
-    r.c = P.rng[4 * (size_t)P.rng_stride + slot];
-    // batch b holds base + (b < rem) photons; slot s takes photon s of each
-    todo = slot < P.base ? P.n_batches : (slot == P.base ? P.rem : 0);
-  }
-  Counters c = {0, 0, 0, 0, 0, 0};
-  Photon g;
-  g.status = kDead;
-  VoxelWalk v;
-  int hop = 0;
-  int mode = todo ? kNeedPrimary : kDone;
-  int32_t dummy_n = 0;
+  Lane L;
+  L.r = Rng{0, 0, 0, 0, 0};
+  L.p = mk(0.f, 0.f, 0.f);
+  L.ray.d = mk(0.f, 0.f, 1.f);
+  L.ray.inv = mk(0.f, 0.f, 1.f);
+  L.e = 0.0f;
+  L.eid = 0;
+  L.mode = kNeedSlot;
+  L.solid = -1;
+  L.hop = 0;
+  L.scatter = 0;
+  L.pend = 0;
+  L.todo = 0;
+  L.slot = 0xffffffffu;
+  VoxGeom G = P.vox0;
+  Counters c = {0, 0, 0, 0, 0};
+  unsigned long long steps = 0;  // wave-uniform
 
   for (;;) {
-    // ---- cold phase: advance every lane to a voxel walk or to the end
-    while (mode == kNeedPrimary || mode == kHop) {
-      if (mode == kNeedPrimary) {
-        if (todo == 0) {
-          mode = kDone;
-          break;
-        }
-        --todo;
-        sample_primary(P, r, g);
-        ++c.primaries;
-        hop = 0;
-        mode = kHop;
-        continue;
-      }
-      if (hop >= GGH_MAX_HOPS) {
-        mode = kNeedPrimary;
-        continue;
-      }
-      ++hop;
-      ++c.hops;
-      if (!find_and_project(P, g)) {
-        mode = kNeedPrimary;
-        continue;
-      }
-      const DevSolid& s = P.solids[g.solid];
-      if (s.type == kVoxel) {
-        voxel_enter(v, g, s, g.solid, g.solid == P.dose_solid);
-        mode = kInVoxel;
-      } else {
-        track_box(P, cs, s, g, r, c);
-        mode = (g.status == kDead) ? kNeedPrimary : kHop;
-      }
+    // ---- cold phase: every stopped lane advances until it is back inside
+    //      a phantom or has no work left
+    while (__any(L.mode != kInVoxel && L.mode != kDone)) {
+      if (L.mode != kInVoxel && L.mode != kDone) cold_step<kSingle>(P, T, L, G, c);
     }
     // ---- hot phase: voxel steps while enough lanes are inside a phantom
-    bool all_done = false;
-    for (;;) {
-      const uint64_t act = __ballot(mode == kInVoxel);
-      const uint64_t live = __ballot(mode != kDone);
-      if (live == 0) {
-        all_done = true;
-        break;
-      }
-      const int na = __popcll(act), nl = __popcll(live);
-      if (na == 0 || na < min(P.hot_threshold, nl)) break;
-      if (mode == kInVoxel) {
-        if (!voxel_step<false>(P, cs, v, g, r, c, nullptr, dummy_n, 0)) {
-          voxel_leave(v, g, P.solids[v.solid]);
-          mode = (g.status == kDead) ? kNeedPrimary : kHop;
-        }
-      }
-    }
-    if (all_done) break;
+    uint64_t act = __ballot(L.mode == kInVoxel);
+    if (act == 0) break;  // every lane is done
+    const int live = __popcll(__ballot(L.mode != kDone));
+    const int need = min(P.hot_threshold, live);
+    do {
+      bool stepped = false;
+      if (L.mode == kInVoxel) stepped = hot_step<NP, kTrack>(P, T, kSingle ? P.vox0 : G, L);
+      steps += __popcll(__ballot(stepped));
+      act = __ballot(L.mode == kInVoxel);
+    } while (__popcll(act) >= need);
   }
 
-  if (valid) {
-    P.rng[0 * (size_t)P.rng_stride + slot] = r.x;
-    P.rng[1 * (size_t)P.rng_stride + slot] = r.y;
-    P.rng[2 * (size_t)P.rng_stride + slot] = r.z;
-    P.rng[3 * (size_t)P.rng_stride + slot] = r.w;
-    P.rng[4 * (size_t)P.rng_stride + slot] = r.c;
-  }
-  uint32_t sums[6] = {c.primaries, c.voxel_steps, c.interactions, c.detector_steps, c.detector_interactions, c.hops};
-  for (int k = 0; k < 6; ++k) {
+  uint32_t sums[5] = {c.primaries, c.interactions, c.detector_steps, c.detector_interactions, c.hops};
+  const int slots[5] = {0, 2, 3, 4, 5};
+  for (int k = 0; k < 5; ++k) {
     uint32_t t = wave_sum(sums[k]);
-    if ((threadIdx.x & 63) == 0 && t) atomicAdd(&P.stats[k], (unsigned long long)t);
+    if (__lane_id() == 0 && t) atomicAdd(&P.stats[slots[k]], (unsigned long long)t);
   }
+  if (__lane_id() == 0 && steps) atomicAdd(&P.stats[1], steps);
 }
 
 }  // namespace
@@ -626,6 +774,8 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(RunParams P) {
 // dose finalisation (src/kernels/ComputeDoseGGEMSVoxelizedSolid.cl:53-121)
 // ---------------------------------------------------------------------------
 namespace {
+
+constexpr int kDoseBlock = 256;
 
 struct DoseParams {
   const double* edep;
@@ -646,13 +796,14 @@ struct DoseParams {
   float* unc;
 };
 
-__global__ __launch_bounds__(kBlock) void dose_kernel(DoseParams D) {
+__global__ __launch_bounds__(kDoseBlock) void dose_kernel(DoseParams D) {
   const int64_t total = (int64_t)D.nd[0] * D.nd[1] * D.nd[2];
   const int slice = D.nd[0] * D.nd[1];
   const float kGy = 1.0e-12f;
   const float kWater = 1.0f * (6.241509704e+21f / 1.e3f);
   const float vol = D.dosel_size.x * D.dosel_size.y * D.dosel_size.z;
-  for (int64_t gid = blockIdx.x * (int64_t)kBlock + threadIdx.x; gid < total; gid += (int64_t)gridDim.x * kBlock) {
+  for (int64_t gid = blockIdx.x * (int64_t)kDoseBlock + threadIdx.x; gid < total;
+       gid += (int64_t)gridDim.x * kDoseBlock) {
     int z = (int)(gid / slice);
     int x = (int)((gid - (int64_t)z * slice) % D.nd[0]);
     int y = (int)((gid - (int64_t)z * slice) / D.nd[0]);
@@ -685,21 +836,14 @@ __global__ void walk_kernel(RunParams P, int solid, int n_rays, const float* pos
                             int32_t* out, int32_t* count) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_rays) return;
-  Photon g;
-  g.p = mk(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
-  g.d = mk(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]);
-  g.e = 1.0f;
-  g.eid = 0;
-  g.status = kAlive;
-  g.scatter = 0;
-  g.solid = solid;
-  g.dist = 0.0f;
-  VoxelWalk v;
-  voxel_enter(v, g, P.solids[solid], solid, 0);
-  Rng r = {0, 0, 0, 0, 0};
-  Counters c = {0, 0, 0, 0, 0, 0};
+  const DevSolid& s = P.solids[solid];
+  f3 p = g2l_pos(s.m, mk(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]));
+  Ray ray;
+  ray.set(g2l_dir(s.m, mk(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2])));
+  const VoxGeom G = s.vox;
   int n = 0;
-  while (voxel_step<true>(P, P.cs4, v, g, r, c, out + (size_t)i * max_steps, n, max_steps)) {
+  int32_t* o = out + (size_t)i * max_steps;
+  while (walk_step(G, p, ray, o, n, max_steps)) {
     if (n > 16 * max_steps) break;
   }
   count[i] = n;
@@ -709,38 +853,31 @@ __global__ void primaries_kernel(RunParams P, int n, float* pos, float* dir, flo
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   Rng r;
-  r.x = P.rng[0 * (size_t)P.rng_stride + s];
-  r.y = P.rng[1 * (size_t)P.rng_stride + s];
-  r.z = P.rng[2 * (size_t)P.rng_stride + s];
-  r.w = P.rng[3 * (size_t)P.rng_stride + s];
-  r.c = P.rng[4 * (size_t)P.rng_stride + s];
-  Photon g;
-  sample_primary(P, r, g);
-  pos[3 * s] = g.p.x; pos[3 * s + 1] = g.p.y; pos[3 * s + 2] = g.p.z;
-  dir[3 * s] = g.d.x; dir[3 * s + 1] = g.d.y; dir[3 * s + 2] = g.d.z;
-  e[s] = g.e;
-  P.rng[0 * (size_t)P.rng_stride + s] = r.x;
-  P.rng[1 * (size_t)P.rng_stride + s] = r.y;
-  P.rng[2 * (size_t)P.rng_stride + s] = r.z;
-  P.rng[3 * (size_t)P.rng_stride + s] = r.w;
-  P.rng[4 * (size_t)P.rng_stride + s] = r.c;
+  load_stream(P, s, r);
+  f3 gp, gd;
+  float energy;
+  int eid;
+  sample_primary(P, P.ebins, r, gp, gd, energy, eid);
+  pos[3 * s] = gp.x; pos[3 * s + 1] = gp.y; pos[3 * s + 2] = gp.z;
+  dir[3 * s] = gd.x; dir[3 * s + 1] = gd.y; dir[3 * s + 2] = gd.z;
+  e[s] = energy;
+  store_stream(P, s, r);
+}
+
+__global__ void neglog_kernel(int n, const uint32_t* sums, float* fast, float* ref) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  fast[i] = neg_log_uniform(sums[i]);
+  ref[i] = -logf(uniform_from_sum(sums[i]));
 }
 
 __global__ void uniforms_kernel(RunParams P, int n_slots, int n, float* out) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n_slots) return;
   Rng r;
-  r.x = P.rng[0 * (size_t)P.rng_stride + s];
-  r.y = P.rng[1 * (size_t)P.rng_stride + s];
-  r.z = P.rng[2 * (size_t)P.rng_stride + s];
-  r.w = P.rng[3 * (size_t)P.rng_stride + s];
-  r.c = P.rng[4 * (size_t)P.rng_stride + s];
+  load_stream(P, s, r);
   for (int k = 0; k < n; ++k) out[(size_t)s * n + k] = r.uniform();
-  P.rng[0 * (size_t)P.rng_stride + s] = r.x;
-  P.rng[1 * (size_t)P.rng_stride + s] = r.y;
-  P.rng[2 * (size_t)P.rng_stride + s] = r.z;
-  P.rng[3 * (size_t)P.rng_stride + s] = r.w;
-  P.rng[4 * (size_t)P.rng_stride + s] = r.c;
+  store_stream(P, s, r);
 }
 
 }  // namespace
@@ -802,6 +939,7 @@ Mat34 to_mat34(const float* m16) {
 
 struct ggh_context {
   int device = 0;
+  int n_cu = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float last_ms = 0.0f;
@@ -810,8 +948,8 @@ struct ggh_context {
   int nbins = 0, n_mat = 0, n_proc = 0, proc[3] = {0, 0, 0};
   float factor = 0.0f;
   std::vector<float> density_host;
-  DevBuf<float> ebins, cs_atom_r, atom_dens, pp, density;
-  DevBuf<float4> cs4;
+  DevBuf<float> ebins, cs_r, cs_atom_r, atom_dens, pp, density;
+  DevBuf<float4> tab_sig, tab_rcp;
   DevBuf<int32_t> n_elem, elem_idx;
   DevBuf<uint8_t> z;
   // solids
@@ -834,8 +972,14 @@ struct ggh_context {
   DevBuf<float> spectrum, cdf;
   // slots
   DevBuf<uint32_t> rng;
+  DevBuf<uint32_t> queue;
   DevBuf<unsigned long long> stats;
-  int hot_threshold = 40;
+  // launch shape
+  int hot_threshold = 32;
+  int blocks_per_cu = 0;  // 0: as many as fit
+  bool force_general = false;
+  unsigned last_grid = 0;
+  int last_variant = -1;
 };
 
 namespace {
@@ -854,6 +998,35 @@ int sync_solids(ggh_context* c) {
   return 0;
 }
 
+// RN(1/v) on the host, for div_rn; 1/0 = +inf keeps sigma == 0 a never-winning process
+float host_rcp(float v) {
+  volatile float one = 1.0f;
+  return one / v;
+}
+
+VoxGeom make_vox(const ggh_voxel_solid_desc* d, int solid) {
+  VoxGeom G;
+  std::memset(&G, 0, sizeof G);
+  G.lo = mk(d->obb.border_min[0], d->obb.border_min[1], d->obb.border_min[2]);
+  G.vs = mk(d->voxel_size[0], d->voxel_size[1], d->voxel_size[2]);
+  G.rvs = mk(host_rcp(G.vs.x), host_rcp(G.vs.y), host_rcp(G.vs.z));
+  G.in_lo = mk(d->obb.border_min[0] + kTol, d->obb.border_min[1] + kTol, d->obb.border_min[2] + kTol);
+  G.in_hi = mk(d->obb.border_max[0] - kTol, d->obb.border_max[1] - kTol, d->obb.border_max[2] - kTol);
+  G.nx = d->n_voxels[0];
+  G.ny = d->n_voxels[1];
+  G.nz = d->n_voxels[2];
+  G.nxy = d->n_voxels[0] * d->n_voxels[1];
+  G.mat_offset = d->material_offset;
+  G.solid = solid;
+  return G;
+}
+
+int n_voxel_solids(const ggh_context* c) {
+  int n = 0;
+  for (const DevSolid& s : c->solids) n += s.type == kVoxel;
+  return n;
+}
+
 RunParams make_params(ggh_context* c) {
   RunParams P;
   std::memset(&P, 0, sizeof P);
@@ -864,7 +1037,9 @@ RunParams make_params(ggh_context* c) {
   P.n_proc = c->n_proc;
   for (int i = 0; i < 3; ++i) P.proc[i] = c->proc[i];
   P.ebins = c->ebins.p;
-  P.cs4 = c->cs4.p;
+  P.tab_sig = c->tab_sig.p;
+  P.tab_rcp = c->tab_rcp.p;
+  P.cs_r = c->cs_r.p;
   P.cs_atom_r = c->cs_atom_r.p;
   P.n_elem = c->n_elem.p;
   P.elem_idx = c->elem_idx.p;
@@ -894,8 +1069,14 @@ RunParams make_params(ggh_context* c) {
   }
   P.rng = c->rng.p;
   P.rng_stride = GGH_MAX_SLOTS;
+  P.queue = c->queue.p;
   P.stats = c->stats.p;
   P.hot_threshold = c->hot_threshold;
+  for (const DevSolid& s : c->solids)
+    if (s.type == kVoxel) {
+      P.vox0 = s.vox;
+      break;
+    }
   return P;
 }
 
@@ -907,6 +1088,10 @@ uint32_t urandom_seed() {
     ::close(fd);
   }
   return s;
+}
+
+size_t tables_lds_bytes(const ggh_context* c) {
+  return (size_t)2 * c->n_mat * c->nbins * sizeof(float4) + (size_t)c->nbins * sizeof(float);
 }
 
 }  // namespace
@@ -930,15 +1115,17 @@ int ggh_context_create(int device, ggh_context** out) {
   ggh_context* c = new (std::nothrow) ggh_context();
   if (!c) return fail(-3, "out of host memory");
   c->device = device;
-  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  hipError_t e = hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
   if (e != hipSuccess) {
     delete c;
-    return fail(-100 - (int)e, std::string("stream/event creation: ") + hipGetErrorString(e));
+    return fail(-100 - (int)e, std::string("device query / stream / event creation: ") + hipGetErrorString(e));
   }
   int rc = dev_alloc(c->stats, 8);
   if (!rc) rc = dev_alloc(c->rng, (size_t)5 * GGH_MAX_SLOTS);
+  if (!rc) rc = dev_alloc(c->queue, 1);
   if (rc) {
     ggh_context_destroy(c);
     return rc;
@@ -952,14 +1139,15 @@ int ggh_context_destroy(ggh_context* c) {
   if (!c) return 0;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  c->ebins.release(); c->cs_atom_r.release(); c->atom_dens.release(); c->pp.release(); c->density.release();
-  c->cs4.release(); c->n_elem.release(); c->elem_idx.release(); c->z.release();
+  c->ebins.release(); c->cs_r.release(); c->cs_atom_r.release(); c->atom_dens.release(); c->pp.release();
+  c->density.release(); c->tab_sig.release(); c->tab_rcp.release(); c->n_elem.release(); c->elem_idx.release();
+  c->z.release();
   for (auto& b : c->labels) b.release();
   for (auto& b : c->hist) b.release();
   for (auto& b : c->scat) b.release();
   c->solids_dev.release();
   c->edep.release(); c->edep2.release(); c->hit.release(); c->ptrack.release();
-  c->spectrum.release(); c->cdf.release(); c->rng.release(); c->stats.release();
+  c->spectrum.release(); c->cdf.release(); c->rng.release(); c->queue.release(); c->stats.release();
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -973,25 +1161,33 @@ int ggh_set_physics(ggh_context* c, const ggh_physics_desc* d) {
   if (!d || d->nbins < 2 || d->nbins > 2048 || d->n_materials < 1 || d->n_materials > 4096)
     return fail(-2, "invalid physics description");
   if (d->n_processes < 0 || d->n_processes > 3) return fail(-2, "invalid number of processes");
-  for (int i = 0; i < d->n_processes; ++i)
+  for (int i = 0; i < d->n_processes; ++i) {
     if (d->process_id[i] < 0 || d->process_id[i] > 2) return fail(-2, "invalid process id");
+    for (int j = 0; j < i; ++j)
+      if (d->process_id[j] == d->process_id[i]) return fail(-2, "process activated twice");
+  }
   const int nb = d->nbins, nm = d->n_materials;
-  std::vector<float4> cs4((size_t)nb * nm);
-  for (int m = 0; m < nm; ++m)
-    for (int i = 0; i < nb; ++i) {
-      float4 v;
-      v.x = d->cross_sections[(size_t)0 * nm * nb + (size_t)m * nb + i];
-      v.y = d->cross_sections[(size_t)1 * nm * nb + (size_t)m * nb + i];
-      v.z = d->cross_sections[(size_t)2 * nm * nb + (size_t)m * nb + i];
-      v.w = d->photon_cut[m];
-      cs4[(size_t)m * nb + i] = v;
-    }
   for (int m = 0; m < nm; ++m)
     if (d->element_index[m] < 0 || d->n_elements[m] < 1 ||
         d->element_index[m] + d->n_elements[m] > d->total_elements)
       return fail(-2, "invalid element table");
   for (int k = 0; k < d->total_elements; ++k)
     if (d->atomic_number[k] < 1 || d->atomic_number[k] > 100) return fail(-2, "atomic number out of range");
+  // step tables: cross sections in activation order, their reciprocals, the cut
+  std::vector<float4> sig((size_t)nb * nm), rcp((size_t)nb * nm);
+  std::vector<float> cs_r((size_t)nb * nm);
+  for (int m = 0; m < nm; ++m)
+    for (int i = 0; i < nb; ++i) {
+      float s[3] = {0.f, 0.f, 0.f}, y[3] = {0.f, 0.f, 0.f};
+      for (int k = 0; k < d->n_processes; ++k) {
+        s[k] = d->cross_sections[(size_t)d->process_id[k] * nm * nb + (size_t)m * nb + i];
+        y[k] = host_rcp(s[k]);
+      }
+      const size_t at = (size_t)m * nb + i;
+      sig[at] = make_float4(s[0], s[1], s[2], d->photon_cut[m]);
+      rcp[at] = make_float4(y[0], y[1], y[2], 0.f);
+      cs_r[at] = d->cross_sections[(size_t)2 * nm * nb + (size_t)m * nb + i];
+    }
   c->nbins = nb;
   c->n_mat = nm;
   c->n_proc = d->n_processes;
@@ -999,7 +1195,9 @@ int ggh_set_physics(ggh_context* c, const ggh_physics_desc* d) {
   c->factor = d->rayleigh_factor;
   c->density_host.assign(d->density, d->density + nm);
   if ((rc = dev_upload(c->ebins, d->energy_bins, (size_t)nb, c->stream))) return rc;
-  if ((rc = dev_upload(c->cs4, cs4.data(), cs4.size(), c->stream))) return rc;
+  if ((rc = dev_upload(c->tab_sig, sig.data(), sig.size(), c->stream))) return rc;
+  if ((rc = dev_upload(c->tab_rcp, rcp.data(), rcp.size(), c->stream))) return rc;
+  if ((rc = dev_upload(c->cs_r, cs_r.data(), cs_r.size(), c->stream))) return rc;
   if ((rc = dev_upload(c->cs_atom_r, d->cs_atom_rayleigh, (size_t)101 * nb, c->stream))) return rc;
   if ((rc = dev_upload(c->n_elem, d->n_elements, (size_t)nm, c->stream))) return rc;
   if ((rc = dev_upload(c->elem_idx, d->element_index, (size_t)nm, c->stream))) return rc;
@@ -1020,6 +1218,8 @@ int ggh_add_voxel_solid(ggh_context* c, const ggh_voxel_solid_desc* d) {
   size_t n = (size_t)d->n_voxels[0] * d->n_voxels[1] * d->n_voxels[2];
   if (d->n_voxels[0] <= 0 || d->n_voxels[1] <= 0 || d->n_voxels[2] <= 0 || n > (size_t)INT32_MAX)
     return fail(-2, "invalid voxel dimensions");
+  for (int i = 0; i < 3; ++i)
+    if (!(d->voxel_size[i] > 0.0f) || !std::isfinite(d->voxel_size[i])) return fail(-2, "invalid voxel size");
   DevSolid s;
   std::memset(&s, 0, sizeof s);
   s.m = to_mat34(d->obb.matrix);
@@ -1027,14 +1227,13 @@ int ggh_add_voxel_solid(ggh_context* c, const ggh_voxel_solid_desc* d) {
   s.border.hi = mk(d->obb.border_max[0], d->obb.border_max[1], d->obb.border_max[2]);
   s.type = kVoxel;
   s.mat_offset = d->material_offset;
-  s.voxel_size = mk(d->voxel_size[0], d->voxel_size[1], d->voxel_size[2]);
-  for (int i = 0; i < 3; ++i) s.nv[i] = d->n_voxels[i];
   int max_label = 0;
   for (size_t i = 0; i < n; ++i) max_label = d->labels[i] > max_label ? d->labels[i] : max_label;
   DevBuf<uint8_t> lab;
   if ((rc = dev_upload(lab, d->labels, n, c->stream))) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream));
-  s.labels = lab.p;
+  s.vox = make_vox(d, (int)c->solids.size());
+  s.vox.labels = lab.p;
   c->labels.push_back(lab);
   c->hist.push_back(DevBuf<int32_t>());
   c->scat.push_back(DevBuf<int32_t>());
@@ -1158,6 +1357,17 @@ int ggh_reset_tallies(ggh_context* c) {
   return 0;
 }
 
+int ggh_set_launch(ggh_context* c, int32_t hot_threshold, int32_t blocks_per_cu, int32_t general_kernel) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (hot_threshold < 0 || hot_threshold > 64 || blocks_per_cu < 0 || blocks_per_cu > 64)
+    return fail(-2, "invalid launch parameters");
+  c->hot_threshold = hot_threshold ? hot_threshold : 32;
+  c->blocks_per_cu = blocks_per_cu;
+  c->force_general = general_kernel != 0;
+  return 0;
+}
+
 int ggh_run(ggh_context* c, uint64_t n) {
   int rc = check_ctx(c);
   if (rc) return rc;
@@ -1174,20 +1384,37 @@ int ggh_run(ggh_context* c, uint64_t n) {
   // batch layout of GGEMSSource::OrganizeParticlesInBatch (src/sources/GGEMSSource.cc:403-425)
   uint64_t nb = (uint64_t)std::ceil((float)n / (float)GGH_MAX_SLOTS);
   if (nb == 0) nb = 1;
-  P.n_batches = nb;
-  P.base = nb == 1 ? n : n / nb;
-  P.rem = nb == 1 ? 0 : n % nb;
-  uint64_t slots = P.base + (P.rem ? 1 : 0);
+  const uint64_t base = nb == 1 ? n : n / nb;
+  const uint64_t rem = nb == 1 ? 0 : n % nb;
+  const uint64_t slots = base + (rem ? 1 : 0);
   if (slots > GGH_MAX_SLOTS) return fail(-3, "batch larger than the slot buffer");
+  if (nb > 0xffffffffull) return fail(-3, "too many batches");
+  P.n_batches = (uint32_t)nb;
+  P.base = (uint32_t)base;
+  P.rem = (uint32_t)rem;
   P.n_slots = (uint32_t)slots;
-  const size_t lds = (size_t)c->n_mat * c->nbins * sizeof(float4);
-  const unsigned grid = (unsigned)((slots + kBlock - 1) / kBlock);
+  const bool single = n_voxel_solids(c) == 1;
+  const size_t lds = tables_lds_bytes(c);
+  const bool use_lds = lds <= (size_t)kLdsBudget;
+  const size_t shmem = use_lds ? lds : 0;
+  // kernel variants: the specialised one (three activated processes, one
+  // voxel solid, tables in LDS, no fluence tracking) and the general one
+  using Kernel = void (*)(RunParams);
+  const bool special = c->n_proc == 3 && single && use_lds && !c->ptrack.p && !c->force_general;
+  const Kernel fn = special ? transport_kernel<3, true, true, false>
+                            : (use_lds ? transport_kernel<-1, false, true, true> : transport_kernel<-1, false, false, true>);
+  c->last_variant = special ? 0 : 1;
+  int per_cu = 0;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn), kBlock, shmem));
+  if (per_cu < 1) return fail(-3, "transport kernel does not fit on a compute unit");
+  if (c->blocks_per_cu > 0) per_cu = std::min(per_cu, c->blocks_per_cu);
+  // persistent blocks: no more than the slots can keep busy
+  const uint64_t max_blocks = (slots + kBlock - 1) / kBlock;
+  const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)per_cu * c->n_cu, max_blocks));
+  c->last_grid = grid;
+  HIP_TRY(hipMemsetAsync(c->queue.p, 0, sizeof(uint32_t), c->stream));
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
-  if (lds <= 48 * 1024) {
-    hipLaunchKernelGGL(transport_kernel<true>, dim3(grid), dim3(kBlock), lds, c->stream, P);
-  } else {
-    hipLaunchKernelGGL(transport_kernel<false>, dim3(grid), dim3(kBlock), 0, c->stream, P);
-  }
+  hipLaunchKernelGGL(fn, dim3(grid), dim3(kBlock), shmem, c->stream, P);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
   HIP_TRY(hipEventSynchronize(c->ev1));
@@ -1278,23 +1505,25 @@ int ggh_compute_dose(ggh_context* c, float scale, int32_t water_ref, float min_d
   D.edep = c->edep.p;
   D.edep2 = c->edep2.p;
   D.hit = c->hit.p;
-  D.labels = s.labels;
+  D.labels = s.vox.labels;
   D.density = c->density.p;
   D.mat_offset = s.mat_offset;
-  for (int i = 0; i < 3; ++i) {
-    D.nd[i] = c->dose_desc.n_dosels[i];
-    D.nv[i] = s.nv[i];
-  }
+  D.nd[0] = c->dose_desc.n_dosels[0];
+  D.nd[1] = c->dose_desc.n_dosels[1];
+  D.nd[2] = c->dose_desc.n_dosels[2];
+  D.nv[0] = s.vox.nx;
+  D.nv[1] = s.vox.ny;
+  D.nv[2] = s.vox.nz;
   D.dosel_size = mk(c->dose_desc.dosel_size[0], c->dose_desc.dosel_size[1], c->dose_desc.dosel_size[2]);
-  D.vs = s.voxel_size;
+  D.vs = s.vox.vs;
   D.vlo = s.border.lo;
   D.scale = scale;
   D.water_reference = water_ref;
   D.min_density = min_density;
   D.dose = d_dose.p;
   D.unc = unc ? d_unc.p : nullptr;
-  unsigned grid = (unsigned)std::min<size_t>((n + kBlock - 1) / kBlock, 65536);
-  hipLaunchKernelGGL(dose_kernel, dim3(grid), dim3(kBlock), 0, c->stream, D);
+  unsigned grid = (unsigned)std::min<size_t>((n + kDoseBlock - 1) / kDoseBlock, 65536);
+  hipLaunchKernelGGL(dose_kernel, dim3(grid), dim3(kDoseBlock), 0, c->stream, D);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(dose, d_dose.p, n * sizeof(float), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess && unc) e = hipMemcpyAsync(unc, d_unc.p, n * sizeof(float), hipMemcpyDeviceToHost, c->stream);
@@ -1366,8 +1595,53 @@ int ggh_debug_uniforms(ggh_context* c, int n_slots, int n, float* out) {
   return 0;
 }
 
+int ggh_debug_neg_log(ggh_context* c, int n, const uint32_t* sums, float* fast, float* ref) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (n <= 0) return fail(-2, "invalid count");
+  DevBuf<uint32_t> ds;
+  DevBuf<float> df, dr;
+  if ((rc = dev_upload(ds, sums, (size_t)n, c->stream)) || (rc = dev_alloc(df, (size_t)n)) ||
+      (rc = dev_alloc(dr, (size_t)n)))
+    return rc;
+  hipLaunchKernelGGL(neglog_kernel, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, ds.p, df.p, dr.p);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(fast, df.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(ref, dr.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  ds.release(); df.release(); dr.release();
+  return 0;
+}
+
+int ggh_last_grid(ggh_context* c, int32_t* blocks, int32_t* block_size, int32_t* variant) {
+  if (!c || !blocks || !block_size || !variant) return fail(-1, "null argument");
+  *blocks = (int32_t)c->last_grid;
+  *block_size = kBlock;
+  *variant = c->last_variant;
+  return 0;
+}
+
 void ggh_host_uniform_from_sum(const uint32_t* sums, float* out, size_t n) {
   for (size_t i = 0; i < n; ++i) out[i] = uniform_from_sum(sums[i]);
+}
+
+void ggh_host_div_rn(const float* a, const float* b, float* out, size_t n) {
+  for (size_t i = 0; i < n; ++i) {
+    volatile float one = 1.0f;
+    const float y = one / b[i];
+    out[i] = div_rn(a[i], b[i], y);
+  }
+}
+
+void ggh_host_distance_to_aabb(const float* p, const float* d, const float* box, float* out, size_t n) {
+  for (size_t i = 0; i < n; ++i) {
+    Ray r;
+    r.set(mk(d[3 * i], d[3 * i + 1], d[3 * i + 2]));
+    Box b;
+    b.lo = mk(box[6 * i], box[6 * i + 1], box[6 * i + 2]);
+    b.hi = mk(box[6 * i + 3], box[6 * i + 4], box[6 * i + 5]);
+    out[i] = distance_to_aabb(mk(p[3 * i], p[3 * i + 1], p[3 * i + 2]), r, b);
+  }
 }
 
 }  // extern "C"

@@ -44,22 +44,39 @@ __host__ __device__ inline f3 normalize(f3 v) {
 // 511 * M > 3 * 2^31 (M >= 12607537) and k = 1 otherwise: the FP64 division
 // and product only decide which neighbour the final rounding lands on.  The
 // identity is checked exhaustively over every float reachable from a uint32.
+// Branch-free form: with t = mantissa bits of f * 2^-32, the two-ulp cases
+// M == 2^23 and M >= 12607537 are exactly (t - 1) >= 12607537 - 2^23 - 1
+// in unsigned arithmetic (t == 0 wraps around).
 __host__ __device__ inline float uniform_from_sum(uint32_t s) {
-  float f = (float)s;
-  if (f == 0.0f) return 0.0f;
-  float x = f * 0x1p-32f;
+  float x = (float)s * 0x1p-32f;
   uint32_t b;
   __builtin_memcpy(&b, &x, 4);
-  uint32_t m = (b & 0x7fffffu) | 0x800000u;
-  b -= (m == 0x800000u || m >= 12607537u) ? 2u : 1u;
+  b -= ((b & 0x7fffffu) - 1u) >= 4218928u ? 2u : 1u;
   float u;
   __builtin_memcpy(&u, &b, 4);
-  return u;
+  return s == 0u ? 0.0f : u;
+}
+
+// a / b rounded to nearest, from y = RN(1/b) computed once on the host:
+// q0 = RN(a y) is within 1.5 ulp of a/b, one correction with the exact
+// remainder r = a - q b (an FMA) makes it faithful, and a second one makes
+// it the correctly rounded quotient (Markstein, IBM J. Res. Dev. 34(1), 1990),
+// for every a/b whose remainders stay normal (|a| >= 2^-100 and a normal
+// quotient here).  When a/b is +inf (a = +inf, or b = 0 with y = +inf) the
+// result is NaN; every caller compares it with `<`, which is false for
+// NaN exactly as for +inf.
+__host__ __device__ inline float div_rn(float a, float b, float y) {
+  float q = a * y;
+  float r = __builtin_fmaf(-q, b, a);
+  q = __builtin_fmaf(r, y, q);
+  r = __builtin_fmaf(-q, b, a);
+  return __builtin_fmaf(r, y, q);
 }
 
 struct Rng {
   uint32_t x, y, z, w, c;
-  __device__ inline float uniform() {
+  // advances the state and returns the 32-bit sum KissUniform maps to [0, 1)
+  __device__ inline uint32_t next_sum() {
     y ^= y << 5;
     y ^= y >> 7;
     y ^= y << 22;
@@ -68,8 +85,9 @@ struct Rng {
     c = t < 0;
     w = (uint32_t)t & 2147483647u;
     x += 1411392427u;
-    return uniform_from_sum(x + y + w);
+    return x + y + w;
   }
+  __device__ inline float uniform() { return uniform_from_sum(next_sum()); }
 };
 
 // ---------------------------------------------------------------------------
@@ -146,7 +164,8 @@ struct Box {
   f3 lo, hi;
 };
 
-__device__ inline void safety_inside(f3& p, const Box& b) {
+// TransportGetSafetyInsideAABB (GGEMSRayTracing.hh:130-153)
+__host__ __device__ inline void safety_inside(f3& p, const Box& b) {
   float smin = fabsf(p.x - b.lo.x), smax = fabsf(p.x - b.hi.x);
   p.x = (smin < kTol) ? b.lo.x + kTol : p.x;
   p.x = (smax < kTol) ? b.hi.x - kTol : p.x;
@@ -160,7 +179,8 @@ __device__ inline void safety_inside(f3& p, const Box& b) {
   p.z = (smax < kTol) ? b.hi.z - kTol : p.z;
 }
 
-__device__ inline void safety_outside(f3& p, const Box& b) {
+// TransportGetSafetyOutsideAABB (GGEMSRayTracing.hh:156-178)
+__host__ __device__ inline void safety_outside(f3& p, const Box& b) {
   float smin = fabsf(p.x - b.lo.x), smax = fabsf(p.x - b.hi.x);
   p.x = (smin < kTol) ? b.lo.x - kTol : p.x;
   p.x = (smax < kTol) ? b.hi.x + kTol : p.x;
@@ -174,7 +194,9 @@ __device__ inline void safety_outside(f3& p, const Box& b) {
   p.z = (smax < kTol) ? b.hi.z + kTol : p.z;
 }
 
-__device__ inline bool in_aabb(f3 p, const Box& b) {
+// IsParticleInAABB (GGEMSRayTracing.hh:60-82): the tolerance argument is
+// ignored there in favour of GEOMETRY_TOLERANCE
+__host__ __device__ inline bool in_aabb(f3 p, const Box& b) {
   if (p.x < (b.lo.x + kTol) || p.x > (b.hi.x - kTol)) return false;
   if (p.y < (b.lo.y + kTol) || p.y > (b.hi.y - kTol)) return false;
   if (p.z < (b.lo.z + kTol) || p.z > (b.hi.z - kTol)) return false;
@@ -185,7 +207,7 @@ __device__ inline bool in_aabb(f3 p, const Box& b) {
 // recomputes on every call) cached until the direction changes.
 struct Ray {
   f3 d, inv;
-  __device__ inline void set(f3 dir) {
+  __host__ __device__ inline void set(f3 dir) {
     d = dir;
     inv.x = 1.0f / dir.x;
     inv.y = 1.0f / dir.y;
@@ -193,43 +215,48 @@ struct Ray {
   }
 };
 
-// ComputeDistanceToAABB (GGEMSRayTracing.hh:223-298)
-__device__ inline float distance_to_aabb(f3 p, const Ray& r, const Box& b) {
-  float tmin = kFltMin, tmax = kOutOfWorld, tmp;
-  if (fabsf(r.d.x) < kEps6) {
-    if (p.x < b.lo.x || p.x > b.hi.x) return kOutOfWorld;
-  } else {
-    tmin = (b.lo.x - p.x) * r.inv.x;
-    tmax = (b.hi.x - p.x) * r.inv.x;
-    if (tmin > tmax) { tmp = tmin; tmin = tmax; tmax = tmp; }
-    if (tmin > tmax) return kOutOfWorld;
-  }
-  if (fabsf(r.d.y) < kEps6) {
-    if (p.y < b.lo.y || p.y > b.hi.y) return kOutOfWorld;
-  } else {
-    float tymin = (b.lo.y - p.y) * r.inv.y;
-    float tymax = (b.hi.y - p.y) * r.inv.y;
-    if (tymin > tymax) { tmp = tymin; tymin = tymax; tymax = tmp; }
-    if (tymin > tmin) tmin = tymin;
-    if (tymax < tmax) tmax = tymax;
-    if (tmin > tmax) return kOutOfWorld;
-  }
-  if (fabsf(r.d.z) < kEps6) {
-    if (p.z < b.lo.z || p.z > b.hi.z) return kOutOfWorld;
-  } else {
-    float tzmin = (b.lo.z - p.z) * r.inv.z;
-    float tzmax = (b.hi.z - p.z) * r.inv.z;
-    if (tzmin > tzmax) { tmp = tzmin; tzmin = tzmax; tzmax = tmp; }
-    if (tzmin > tmin) tmin = tzmin;
-    if (tzmax < tmax) tmax = tzmax;
-    if (tmin > tmax) return kOutOfWorld;
-  }
-  if (tmin < 0.0f && (tmax < 0.0f || tmax == 0.0f)) return kOutOfWorld;
-  // (tmax-tmin) < 2.0*tolerance is evaluated in FP64 by the reference; 2*kTol
-  // is exactly representable in FP32, so the FP32 comparison is identical.
-  if ((tmax - tmin) < 2.0f * kTol) return kOutOfWorld;
-  if (tmin <= kEps6) return tmax;
-  return tmin;
+// ComputeDistanceToAABB (GGEMSRayTracing.hh:223-298) without branches.
+// The reference returns OUT_OF_WORLD at the first failing test; here every
+// test is evaluated and OR-ed, which returns the same value because no test
+// has side effects.  Per axis:
+//   |d| < 1e-6: parallel, out when p lies outside the slab;
+//   otherwise the slab interval [min(t1,t2), max(t1,t2)] (the reference's
+//   compare-and-swap) replaces [tmin, tmax] on x and narrows it on y and z,
+//   and an empty interval is out.
+// The products of a parallel axis can be NaN (0 * inf); they are never selected.
+// (tmax - tmin) < 2*tol is evaluated in FP64 by the reference; 2*kTol is
+// exactly representable in FP32, so the FP32 comparison is identical.
+__host__ __device__ inline float distance_to_aabb(f3 p, const Ray& r, const Box& b) {
+  // bitwise & and | on the tests keep the evaluation free of branches
+  bool par = fabsf(r.d.x) < kEps6;
+  float t1 = (b.lo.x - p.x) * r.inv.x, t2 = (b.hi.x - p.x) * r.inv.x;
+  float lo = t1 > t2 ? t2 : t1, hi = t1 > t2 ? t1 : t2;
+  float tmin = par ? kFltMin : lo;
+  float tmax = par ? kOutOfWorld : hi;
+  bool out = par & ((p.x < b.lo.x) | (p.x > b.hi.x));
+
+  par = fabsf(r.d.y) < kEps6;
+  t1 = (b.lo.y - p.y) * r.inv.y;
+  t2 = (b.hi.y - p.y) * r.inv.y;
+  lo = t1 > t2 ? t2 : t1;
+  hi = t1 > t2 ? t1 : t2;
+  tmin = (!par & (lo > tmin)) ? lo : tmin;
+  tmax = (!par & (hi < tmax)) ? hi : tmax;
+  out |= par ? ((p.y < b.lo.y) | (p.y > b.hi.y)) : (tmin > tmax);
+
+  par = fabsf(r.d.z) < kEps6;
+  t1 = (b.lo.z - p.z) * r.inv.z;
+  t2 = (b.hi.z - p.z) * r.inv.z;
+  lo = t1 > t2 ? t2 : t1;
+  hi = t1 > t2 ? t1 : t2;
+  tmin = (!par & (lo > tmin)) ? lo : tmin;
+  tmax = (!par & (hi < tmax)) ? hi : tmax;
+  out |= par ? ((p.z < b.lo.z) | (p.z > b.hi.z)) : (tmin > tmax);
+
+  out |= (tmin < 0.0f) & (tmax <= 0.0f);   // tmax < 0 || tmax == 0
+  out |= (tmax - tmin) < 2.0f * kTol;
+  const float d = tmin <= kEps6 ? tmax : tmin;
+  return out ? kOutOfWorld : d;
 }
 
 }  // namespace ggh

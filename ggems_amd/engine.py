@@ -100,6 +100,11 @@ def library():
             "ggh_debug_primaries": ([vp, C.c_int, vp, vp, vp], C.c_int),
             "ggh_debug_uniforms": ([vp, C.c_int, C.c_int, vp], C.c_int),
             "ggh_host_uniform_from_sum": ([vp, vp, C.c_size_t], None),
+            "ggh_host_div_rn": ([vp, vp, vp, C.c_size_t], None),
+            "ggh_host_distance_to_aabb": ([vp, vp, vp, vp, C.c_size_t], None),
+            "ggh_set_launch": ([vp, i32, i32, i32], C.c_int),
+            "ggh_last_grid": ([vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)], C.c_int),
+            "ggh_debug_neg_log": ([vp, C.c_int, vp, vp, vp], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -123,6 +128,25 @@ def host_uniform_from_sum(sums):
     sums = np.ascontiguousarray(sums, dtype=np.uint32)
     out = np.empty(sums.shape, np.float32)
     library().ggh_host_uniform_from_sum(_ptr(sums), _ptr(out), sums.size)
+    return out
+
+
+def host_div_rn(a, b):
+    """a / b by the engine's division-free sequence, host build."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    b = np.ascontiguousarray(b, dtype=np.float32)
+    out = np.empty(a.shape, np.float32)
+    library().ggh_host_div_rn(_ptr(a), _ptr(b), _ptr(out), a.size)
+    return out
+
+
+def host_distance_to_aabb(p, d, box):
+    """The engine's slab test, host build: p, d [n, 3]; box [n, 6] = min xyz, max xyz."""
+    p = np.ascontiguousarray(p, dtype=np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(d, dtype=np.float32).reshape(-1, 3)
+    box = np.ascontiguousarray(box, dtype=np.float32).reshape(-1, 6)
+    out = np.empty(p.shape[0], np.float32)
+    library().ggh_host_distance_to_aabb(_ptr(p), _ptr(d), _ptr(box), _ptr(out), p.shape[0])
     return out
 
 
@@ -318,6 +342,22 @@ class Engine:
         e = np.zeros(n, np.float32)
         _check(library().ggh_debug_primaries(self.h, n, _ptr(pos), _ptr(dir), _ptr(e)))
         return pos, dir, e
+
+    def set_launch(self, hot_threshold=0, blocks_per_cu=0, general_kernel=False):
+        _check(library().ggh_set_launch(self.h, int(hot_threshold), int(blocks_per_cu), int(bool(general_kernel))))
+
+    def last_grid(self):
+        """(blocks, block size, kernel variant: 0 specialised, 1 general) of the last run"""
+        b, s, v = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(library().ggh_last_grid(self.h, C.byref(b), C.byref(s), C.byref(v)))
+        return b.value, s.value, v.value
+
+    def debug_neg_log(self, sums):
+        sums = np.ascontiguousarray(sums, dtype=np.uint32)
+        fast = np.empty(sums.shape, np.float32)
+        ref = np.empty(sums.shape, np.float32)
+        _check(library().ggh_debug_neg_log(self.h, sums.size, _ptr(sums), _ptr(fast), _ptr(ref)))
+        return fast, ref
 
     def debug_uniforms(self, n_slots, n):
         out = np.zeros((n_slots, n), np.float32)

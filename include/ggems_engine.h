@@ -148,6 +148,15 @@ int ggh_last_kernel_ms(ggh_context* ctx, float* ms);
 int ggh_reset_tallies(ggh_context* ctx);
 int ggh_get_stats(ggh_context* ctx, ggh_stats* out);
 
+/* launch shape of ggh_run (no counterpart in GGEMS; for tuning and tests):
+ * a wave keeps stepping voxels while at least hot_threshold of its 64 lanes
+ * are inside a phantom (0 = default 32); blocks_per_cu caps the persistent
+ * blocks per compute unit (0 = as many as fit); general_kernel != 0 runs
+ * the general kernel variant even where the specialised one applies */
+int ggh_set_launch(ggh_context* ctx, int32_t hot_threshold, int32_t blocks_per_cu, int32_t general_kernel);
+/* grid and kernel variant (0 specialised, 1 general) of the last ggh_run */
+int ggh_last_grid(ggh_context* ctx, int32_t* blocks, int32_t* block_size, int32_t* variant);
+
 int ggh_get_histogram(ggh_context* ctx, int solid, int32_t* counts, int32_t* scatter);
 int ggh_get_dose_tallies(ggh_context* ctx, double* edep, double* edep_squared, int32_t* hit,
                          int32_t* photon_tracking);
@@ -163,8 +172,14 @@ int ggh_debug_voxel_walk(ggh_context* ctx, int solid, int n_rays, const float* p
 int ggh_debug_primaries(ggh_context* ctx, int n, float* pos, float* dir, float* energy);
 /* n draws from slot 0..n_slots-1 streams: out[s*n + k] */
 int ggh_debug_uniforms(ggh_context* ctx, int n_slots, int n, float* out);
-/* host implementation of the engine's uniform mapping (exact-equivalence checks) */
+/* -log(u) of the voxel step (fast[i]) and -logf(u) (ref[i]) on the device for RNG sums */
+int ggh_debug_neg_log(ggh_context* ctx, int n, const uint32_t* sums, float* fast, float* ref);
+/* host builds of the engine's arithmetic (exact-equivalence checks against the oracle) */
 void ggh_host_uniform_from_sum(const uint32_t* sums, float* out, size_t n);
+/* a[i] / b[i] by the engine's division-free sequence */
+void ggh_host_div_rn(const float* a, const float* b, float* out, size_t n);
+/* the engine's slab test: p, d [n][3], box [n][6] = {min xyz, max xyz} */
+void ggh_host_distance_to_aabb(const float* p, const float* d, const float* box, float* out, size_t n);
 
 #ifdef __cplusplus
 }

@@ -102,10 +102,15 @@ def test_voxel_walk_bit_exact():
     eng.close()
 
 
-def test_cfg1_run_matches_oracle(cfg1):
+@pytest.mark.parametrize("general", [False, True])
+def test_cfg1_run_matches_oracle(cfg1, general):
+    """The same seeded run through the specialised and the general kernel."""
     n = 200_000
     ref = cfg1.run_oracle(n, 777, threads=0)
-    got = cfg1.run_engine(n, 777)
+    eng = cfg1.make_engine()
+    eng.set_launch(general_kernel=general)
+    got = cfg1.run_engine(n, 777, eng)
+    assert eng.last_grid()[2] == (1 if general else 0)
     rs, gs = ref["stats"], got["stats"]
     assert gs["primaries"] == rs["primaries"] == n
     for k in ("voxel_steps", "interactions", "detector_interactions", "hops"):
@@ -120,3 +125,72 @@ def test_cfg1_run_matches_oracle(cfg1):
     e_ref, e_got = ref["edep"], got["edep"]
     assert abs(e_got.sum() - e_ref.sum()) <= 0.01 * e_ref.sum()
     assert np.abs(ref["hit"].astype(np.int64) - got["hit"]).sum() <= 0.02 * ref["hit"].sum()
+    eng.close()
+
+
+def _chi2_two_sample(a, b):
+    """chi-square of two independent Poisson samples of one distribution, p-value."""
+    from scipy.stats import chi2
+    a = a.astype(np.float64).ravel()
+    b = b.astype(np.float64).ravel()
+    m = (a + b) > 0
+    x2 = float(np.sum((a[m] - b[m]) ** 2 / (a[m] + b[m])))
+    return x2, int(m.sum()), chi2.sf(x2, int(m.sum()))
+
+
+def test_cfg1_statistical_agreement_independent_seeds(cfg1):
+    """Engine and oracle with different seeds: per-pixel (8x8 binned) detector
+    counts, scatter counts and per-slab dose agree as samples of one
+    distribution (two-sample chi-square), and the mean z-score is ~0."""
+    n = 400_000
+    ref = cfg1.run_oracle(n, 1001, threads=0)
+    got = cfg1.run_engine(n, 2002)
+
+    def binned(h, k=8):
+        return h.reshape(h.shape[0] // k, k, h.shape[1] // k, k).sum(axis=(1, 3))
+
+    for key in ("hist", "scatter"):
+        a, b = binned(ref[key][0]), binned(got[key][0])
+        x2, dof, p = _chi2_two_sample(a, b)
+        assert p > 1e-4, (key, x2, dof)
+        z = (b.astype(np.float64) - a) / np.sqrt(np.maximum(a + b, 1))
+        assert abs(z.mean()) < 4.0 / np.sqrt(z.size)
+    # hits per 4-voxel slab along x: counts of energy-deposit events
+    ha = ref["hit"].reshape(ref["hit"].shape[0], ref["hit"].shape[1], -1, 4).sum(axis=(0, 1, 3))
+    hb = got["hit"].reshape(got["hit"].shape[0], got["hit"].shape[1], -1, 4).sum(axis=(0, 1, 3))
+    x2, dof, p = _chi2_two_sample(ha, hb)
+    assert p > 1e-4, (x2, dof)
+
+
+def test_step_log_matches_device_logf(cfg1):
+    """The voxel step's -log(u) is bit-identical to -logf(u) on the device."""
+    eng = cfg1.make_engine()
+    rng = np.random.default_rng(9)
+    sums = np.concatenate([rng.integers(1, 2 ** 32, 4_000_000, dtype=np.uint64).astype(np.uint32),
+                           np.arange(1, 200_000, dtype=np.uint32),
+                           (2 ** 32 - 1 - np.arange(200_000)).astype(np.uint32)])
+    fast, ref = eng.debug_neg_log(sums)
+    np.testing.assert_array_equal(fast, ref)
+    f0, r0 = eng.debug_neg_log(np.zeros(1, np.uint32))
+    assert np.isnan(f0[0]) and r0[0] == np.inf  # both lose every comparison with `<`
+    eng.close()
+
+
+def test_engine_reproduces_golden_vectors():
+    import os
+    gold = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_vectors.npz"),
+                   allow_pickle=False)
+    sc = cfg1_scene(n_side=100, detector=256, dosimetry=False)
+    eng = sc.make_engine()
+    eng.seed(777)
+    np.testing.assert_array_equal(eng.debug_uniforms(4, 64), gold["kiss_777"])
+    eng.seed(777)
+    pos, dirs, en = eng.debug_primaries(64)
+    np.testing.assert_array_equal(en, gold["primary_e"])
+    np.testing.assert_allclose(dirs, gold["primary_dir"], atol=2e-7, rtol=0)
+    np.testing.assert_allclose(pos, gold["primary_pos"], atol=1e-4, rtol=0)
+    walks, counts = eng.debug_voxel_walk(0, gold["walk_p"], gold["walk_d"], max_steps=400)
+    for k in range(len(walks)):
+        exp = gold["walks"][k]
+        np.testing.assert_array_equal(walks[k], exp[exp >= 0])
+    eng.close()

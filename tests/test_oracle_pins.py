@@ -67,14 +67,10 @@ def _kiss_literal(s):
 
 def _kiss_bits(s):
     """The division-free mapping of ggems_amd/csrc/engine/transport_device.hpp (uniform_from_sum)."""
-    f = np.float32(np.uint32(s))
-    if f == 0:
-        return np.float32(0.0)
-    x = np.float32(f * np.float32(2.0 ** -32))
-    b = np.array([x], np.float32).view(np.uint32)[0]
-    m = (int(b) & 0x7FFFFF) | 0x800000
-    b = np.uint32(int(b) - (2 if (m == 0x800000 or m >= 12607537) else 1))
-    return np.array([b], np.uint32).view(np.float32)[0]
+    x = np.float32(np.float32(np.uint32(s)) * np.float32(2.0 ** -32))
+    b = int(np.array([x], np.float32).view(np.uint32)[0])
+    b -= 2 if ((b & 0x7FFFFF) - 1) % (2 ** 32) >= 4218928 else 1
+    return np.float32(0.0) if s == 0 else np.array([b], np.uint32).view(np.float32)[0]
 
 
 def test_kiss_uniform_mapping_matches_literal_formula():
