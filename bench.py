@@ -24,6 +24,10 @@ sys.path.insert(0, REPO)
 
 METRIC = "primary photons/s on 1 MI355X (voxel CT); scatter error vs 1e9-photon MC"
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak of one MI355X: the published FP32 vector rate, 157.3 TFLOP/s
+# = 256 CUs x 4 SIMDs x 2.4 GHz x 64 FLOP per cycle, is one wave64 FMA
+# (128 FLOP) per SIMD every 2 cycles
+VALU_PEAK_INSTR_PER_S = 256 * 4 * 2.4e9 / 2
 
 
 def algorithmic_bytes_per_primary(stats):
@@ -78,16 +82,14 @@ def cpu_baseline(workdir, n_per_step, seed, target_s, threads):
     }
 
 
-def measured_traffic(photons_per_step):
-    """HBM bytes per launch from the committed PMC summary of this engine source, else None."""
+def profile_record():
+    """The committed PMC summary of the transport kernel, if it was measured on this engine source."""
     path = os.path.join(REPO, "profiles", "transport_kernel_voxel_ct.json")
     try:
         with open(path) as f:
             rec = json.load(f)
         from ggems_amd.profiling import source_digest
-        if rec.get("source_sha256") != source_digest(REPO) or not rec.get("traffic_bytes_per_primary"):
-            return None
-        return rec["traffic_bytes_per_primary"] * photons_per_step
+        return rec if rec.get("source_sha256") == source_digest(REPO) else None
     except (OSError, ValueError, KeyError):
         return None
 
@@ -110,8 +112,8 @@ def main():
                     help="also run a plain device copy of this size (calibrates profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU; every box here has one MI355X "
-              "(multi-GPU sharding of the photon budget is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py: measures one GPU (--gpus 1); the photon budget of a run shards across GPUs by "
+              "particle slots, with one sum-reduction of the tallies at the end", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -149,7 +151,10 @@ def main():
     bpp, s, i, d = algorithmic_bytes_per_primary(stats)
     avg_kernel_s = sum(kernel_ms) / len(kernel_ms) * 1e-3
     achieved = bpp * args.photons_per_step / avg_kernel_s / 1e9
-    traffic = measured_traffic(args.photons_per_step)
+    prof = profile_record()
+    traffic = prof["traffic_bytes_per_primary"] * args.photons_per_step if prof and prof.get(
+        "traffic_bytes_per_primary") else None
+    steps_per_launch = stats["voxel_steps"] / args.steps
     out = {
         "metric": METRIC,
         "value": value,
@@ -186,6 +191,18 @@ def main():
             "bytes_per_primary": bpp,
         },
     }
+    if prof and prof.get("valu_instr_per_step"):
+        # issue-rate roofline of the same kernel (VALU wave instructions per voxel
+        # step from the committed SQ counters of this source)
+        valu = prof["valu_instr_per_step"] * steps_per_launch / avg_kernel_s
+        out["valu_roofline"] = {
+            "bound": "valu",
+            "achieved": valu,
+            "peak": VALU_PEAK_INSTR_PER_S,
+            "unit": "wave instructions/s",
+            "frac": valu / VALU_PEAK_INSTR_PER_S,
+            "instr_per_voxel_step": prof["valu_instr_per_step"],
+        }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workdir, args.photons_per_step, args.seed, args.cpu_seconds,
                                            args.cpu_threads)

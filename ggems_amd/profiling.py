@@ -82,7 +82,7 @@ def source_digest(repo=None):
     return h.hexdigest()
 
 
-def summarize(kt, fetch, write, sq=None, copy_mb=256, photons=None, kernel=KERNEL):
+def summarize(kt, fetch, write, sq=None, copy_mb=256, photons=None, kernel=KERNEL, extra=(), steps=None):
     copy_bytes = copy_mb * (1 << 20)
     calls, avg, lo, hi = kernel_durations(kt, kernel)
     ff, fd, fkib = copy_factor(fetch, "FETCH_SIZE", copy_bytes)
@@ -112,11 +112,18 @@ def summarize(kt, fetch, write, sq=None, copy_mb=256, photons=None, kernel=KERNE
     }
     if photons:
         out["traffic_bytes_per_primary"] = out["traffic_bytes"] / photons
-    if sq:
-        con = _db(sq)
+    counters = {}
+    for path in ([sq] if sq else []) + list(extra):
+        con = _db(path)
         rows = con.execute("select counter_name, avg(value) from counters_collection where kernel_name like ? "
                            "group by counter_name", ("%" + kernel + "%",)).fetchall()
-        out["sq"] = {r[0]: r[1] for r in rows}
+        counters.update({r[0]: r[1] for r in rows})
+    if counters:
+        out["sq"] = counters
+        hit, miss = counters.get("TCC_HIT_sum"), counters.get("TCC_MISS_sum")
+        if hit is not None and miss is not None and hit + miss > 0:
+            out["l2_hit_rate"] = hit / (hit + miss)
+    out["steps"] = steps
     return out
 
 
@@ -142,13 +149,18 @@ def format_summary(s, commands=()):
         lines.append("primaries per dispatch: %d   traffic per primary: %.1f B"
                      % (s["photons"], s["traffic_bytes_per_primary"]))
     if s.get("sq"):
-        lines.append("SQ counters per dispatch:")
+        lines.append("counters per dispatch:")
         for k in sorted(s["sq"]):
             v = s["sq"][k]
             extra = ""
             if s.get("photons") and k.startswith("SQ_INSTS"):
-                extra = "   (%.1f per primary)" % (v / s["photons"])
-            lines.append("  %-18s %.6g%s" % (k, v, extra))
+                extra = "   (%.1f per primary" % (v / s["photons"])
+                if s.get("steps"):
+                    extra += ", %.3f per voxel step" % (v / s["steps"])
+                extra += ")"
+            lines.append("  %-20s %.6g%s" % (k, v, extra))
+        if "l2_hit_rate" in s:
+            lines.append("L2 hit rate (TCC_HIT / (TCC_HIT + TCC_MISS)): %.3f" % s["l2_hit_rate"])
     if commands:
         lines.append("")
         lines.append("commands:")
@@ -162,6 +174,8 @@ def main():
     ap.add_argument("--fetch", required=True)
     ap.add_argument("--write", required=True)
     ap.add_argument("--sq")
+    ap.add_argument("--extra", action="append", default=[], help="further PMC run directories")
+    ap.add_argument("--steps", type=float, help="voxel steps per dispatch (per-step instruction counts)")
     ap.add_argument("--copy-mb", type=int, default=256)
     ap.add_argument("--photons", type=int)
     ap.add_argument("--kernel", default=KERNEL)
@@ -169,7 +183,8 @@ def main():
     ap.add_argument("--json")
     ap.add_argument("--command", action="append", default=[])
     args = ap.parse_args()
-    s = summarize(args.kt, args.fetch, args.write, args.sq, args.copy_mb, args.photons, args.kernel)
+    s = summarize(args.kt, args.fetch, args.write, args.sq, args.copy_mb, args.photons, args.kernel, args.extra,
+                  args.steps)
     text = format_summary(s, args.command)
     print(text, end="")
     if args.out:
@@ -180,6 +195,12 @@ def main():
                "traffic_bytes_per_primary": s.get("traffic_bytes_per_primary"),
                "fetch_factor": s["fetch_factor"], "write_factor": s["write_factor"],
                "avg_ms": s["avg_ns"] * 1e-6, "source_sha256": source_digest()}
+        c = s.get("sq", {})
+        if args.steps and "SQ_INSTS_VALU" in c:
+            rec["valu_instr_per_step"] = c["SQ_INSTS_VALU"] / args.steps
+            rec["salu_instr_per_step"] = c.get("SQ_INSTS_SALU", 0.0) / args.steps
+        if "l2_hit_rate" in s:
+            rec["l2_hit_rate"] = s["l2_hit_rate"]
         with open(args.json, "w") as f:
             json.dump(rec, f, indent=1)
             f.write("\n")
