@@ -44,7 +44,10 @@ using namespace ggh;
 namespace {
 
 constexpr int kBlock = 512;            // 8 waves; 2 blocks per CU at 4 waves per SIMD
-constexpr int kWavesPerEU = 4;         // register budget: 128 VGPRs per lane
+#ifndef GGH_WAVES_PER_EU
+#define GGH_WAVES_PER_EU 4
+#endif
+constexpr int kWavesPerEU = GGH_WAVES_PER_EU;  // register budget: 512 / waves VGPRs per lane
 constexpr int kLdsBudget = 64 * 1024;  // step tables + energy bins staged per block
 
 // ---------------------------------------------------------------------------
@@ -76,6 +79,8 @@ struct DevSolid {
   int32_t* histogram;
   int32_t* scatter;
 };
+
+static_assert(sizeof(DevSolid) % 16 == 0, "solids are staged in LDS after 16-byte table entries");
 
 struct RunParams {
   const DevSolid* solids;
@@ -120,6 +125,8 @@ struct RunParams {
   uint32_t* queue;            // next slot to hand out
   unsigned long long* stats;  // ggh_stats fields
   int32_t hot_threshold;
+  int32_t resume_threshold;
+  int32_t interact_batch;
   VoxGeom vox0;               // the only voxel solid, when there is exactly one
 };
 
@@ -128,6 +135,9 @@ struct Tabs {
   const float4* sig;
   const float4* rcp;
   const float* ebins;
+  const DevSolid* solids;
+  const float* spectrum;
+  const float* cdf;
 };
 
 // ---------------------------------------------------------------------------
@@ -518,8 +528,8 @@ __device__ void track_box(const RunParams& P, const Tabs& T, const DevSolid& s, 
 // ---------------------------------------------------------------------------
 // source (src/kernels/GetPrimariesGGEMSXRaySource.cl:52-143)
 // ---------------------------------------------------------------------------
-__device__ void sample_primary(const RunParams& P, const float* ebins, Rng& r, f3& pos, f3& dir,
-                                            float& energy, int& eid) {
+__device__ void sample_primary(const RunParams& P, const Tabs& T, Rng& r, f3& pos, f3& dir, float& energy,
+                               int& eid) {
   double phi = r.uniform();
   double theta = r.uniform();
   phi *= (double)kTwoPi;
@@ -538,13 +548,13 @@ __device__ void sample_primary(const RunParams& P, const float* ebins, Rng& r, f
   float rnd = 0.0f;
   if (P.n_energy_arg != 1) {
     rnd = r.uniform();
-    idx = binary_search(rnd, P.cdf, P.n_energy_arg + 1);
+    idx = binary_search(rnd, T.cdf, P.n_energy_arg + 1);
   }
-  energy = (!P.is_interp) ? P.spectrum[idx]
-                          : lin_interp(P.cdf[idx], P.spectrum[idx], P.cdf[idx + 1], P.spectrum[idx + 1], rnd);
+  energy = (!P.is_interp) ? T.spectrum[idx]
+                          : lin_interp(T.cdf[idx], T.spectrum[idx], T.cdf[idx + 1], T.spectrum[idx + 1], rnd);
   pos = gp;
   dir = d;
-  eid = binary_search(energy, ebins, P.nbins);
+  eid = binary_search(energy, T.ebins, P.nbins);
 }
 
 // ---------------------------------------------------------------------------
@@ -554,11 +564,11 @@ __device__ void sample_primary(const RunParams& P, const float* ebins, Rng& r, f
 // (src/kernels/ParticleSolidDistanceGGEMSVoxelizedSolid.cl:43-108) followed by
 // project_to_* (src/kernels/ProjectToGGEMSVoxelizedSolid.cl:44-129).  Returns
 // the solid the photon now sits in, or -1 when it leaves the world.
-__device__ int find_and_project(const RunParams& P, f3& pos, f3 dir) {
+__device__ int find_and_project(const RunParams& P, const Tabs& T, f3& pos, f3 dir) {
   float dist = kOutOfWorld;
   int solid = -1;
   for (int k = 0; k < P.n_solids; ++k) {
-    const DevSolid& s = P.solids[k];
+    const DevSolid& s = T.solids[k];
     f3 lp = g2l_pos(s.m, pos);
     if (in_aabb(lp, s.border)) {
       dist = 0.0f;
@@ -574,7 +584,7 @@ __device__ int find_and_project(const RunParams& P, f3& pos, f3 dir) {
     }
   }
   if (solid == -1 || dist == kOutOfWorld) return -1;
-  const DevSolid& s = P.solids[solid];
+  const DevSolid& s = T.solids[solid];
   pos = add(pos, scl(dir, dist + kTol));
   f3 lp = g2l_pos(s.m, pos);
   safety_inside(lp, s.border);
@@ -603,35 +613,39 @@ __device__ inline void store_stream(const RunParams& P, uint32_t slot, const Rng
   P.rng[4 * (size_t)P.rng_stride + slot] = r.c;
 }
 
+// The discrete process a voxel step stopped on, then the energy cut
+// (TrackThroughGGEMSVoxelizedSolid.cl:238-310).
+__device__ inline void resolve_interaction(const RunParams& P, const Tabs& T, Lane& L, Counters& c) {
+  const int proc = L.pend & 3, gmat = L.pend >> 2;
+  ++c.interactions;
+  const float e0 = L.e;
+  f3 dir = L.ray.d;
+  bool alive = discrete(P, T.ebins, proc, gmat, L.e, L.eid, dir, L.r);
+  if (proc == kCompton || proc == kRayleigh) L.scatter = 1;
+  const bool dose = L.solid == P.dose_solid;
+  if (dose) dose_record(P, e0 - L.e, L.p);
+  if (proc != kPhotoelectric) {
+    L.ray.set(dir);
+    if (L.e != e0) L.eid = binary_search(L.e, T.ebins, P.nbins);
+  }
+  if (L.e <= T.sig[gmat * P.nbins].w) {  // photon_energy_cut_[material]
+    if (dose) dose_record(P, L.e, L.p);
+    alive = false;
+  }
+  L.mode = alive ? kInVoxel : kNeedPrimary;
+}
+
 // the cold work of one lane, one state at a time in an order that lets a
 // lane pass through several states in one call
 template <bool kSingle>
 __device__ inline void cold_step(const RunParams& P, const Tabs& T, Lane& L, VoxGeom& G, Counters& c) {
-  if (L.mode == kInteract) {
-    const int proc = L.pend & 3, gmat = L.pend >> 2;
-    ++c.interactions;
-    const float e0 = L.e;
-    f3 dir = L.ray.d;
-    bool alive = discrete(P, T.ebins, proc, gmat, L.e, L.eid, dir, L.r);
-    if (proc == kCompton || proc == kRayleigh) L.scatter = 1;
-    const bool dose = L.solid == P.dose_solid;
-    if (dose) dose_record(P, e0 - L.e, L.p);
-    if (proc != kPhotoelectric) {
-      L.ray.set(dir);
-      if (L.e != e0) L.eid = binary_search(L.e, T.ebins, P.nbins);
-    }
-    if (L.e <= T.sig[gmat * P.nbins].w) {  // photon_energy_cut_[material]
-      if (dose) dose_record(P, L.e, L.p);
-      alive = false;
-    }
-    L.mode = alive ? kInVoxel : kNeedPrimary;
-  }
+  if (L.mode == kInteract) resolve_interaction(P, T, L, c);
   if (L.mode == kCut) {
     if (L.solid == P.dose_solid) dose_record(P, L.e, L.p);
     L.mode = kNeedPrimary;
   }
   if (L.mode == kExit) {
-    const DevSolid& s = P.solids[L.solid];
+    const DevSolid& s = T.solids[L.solid];
     L.p = l2g_pos(s.m, L.p);
     L.ray.d = l2g_dir(s.m, L.ray.d);
     L.mode = kHop;
@@ -666,7 +680,7 @@ __device__ inline void cold_step(const RunParams& P, const Tabs& T, Lane& L, Vox
       L.mode = kNeedSlot;  // empty slot: take another one on the next pass
     } else {
       --L.todo;
-      sample_primary(P, T.ebins, L.r, L.p, L.ray.d, L.e, L.eid);
+      sample_primary(P, T, L.r, L.p, L.ray.d, L.e, L.eid);
       ++c.primaries;
       L.scatter = 0;
       L.hop = 0;
@@ -680,13 +694,13 @@ __device__ inline void cold_step(const RunParams& P, const Tabs& T, Lane& L, Vox
     }
     ++L.hop;
     ++c.hops;
-    const int k = find_and_project(P, L.p, L.ray.d);
+    const int k = find_and_project(P, T, L.p, L.ray.d);
     if (k < 0) {
       L.mode = kNeedPrimary;
       return;
     }
     L.solid = k;
-    const DevSolid& s = P.solids[k];
+    const DevSolid& s = T.solids[k];
     if (s.type == kVoxel) {
       L.p = g2l_pos(s.m, L.p);
       L.ray.set(g2l_dir(s.m, L.ray.d));
@@ -706,21 +720,38 @@ template <int NP, bool kSingle, bool kLDS, bool kTrack>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWavesPerEU)))
 void transport_kernel(RunParams P) {
   extern __shared__ float4 s_tab[];
-  Tabs T = {P.tab_sig, P.tab_rcp, P.ebins};
+  Tabs T = {P.tab_sig, P.tab_rcp, P.ebins, P.solids, P.spectrum, P.cdf};
   if (kLDS) {
+    // LDS layout (the host sizes it with the same arithmetic, lds_layout):
+    // step tables, solids, energy bins, spectrum, cdf
     const int n = P.n_mat * P.nbins;
     float4* s_sig = s_tab;
     float4* s_rcp = s_tab + n;
-    float* s_eb = reinterpret_cast<float*>(s_tab + 2 * n);
+    DevSolid* s_sol = reinterpret_cast<DevSolid*>(s_tab + 2 * n);
+    float* s_eb = reinterpret_cast<float*>(s_sol + P.n_solids);
+    const int ne = P.n_energy_arg + 1;  // spectrum and cdf entries, with the padding one
+    float* s_spec = s_eb + P.nbins;
+    float* s_cdf = s_spec + ne;
     for (int i = threadIdx.x; i < n; i += kBlock) {
       s_sig[i] = P.tab_sig[i];
       s_rcp[i] = P.tab_rcp[i];
     }
+    const int sw = P.n_solids * (int)(sizeof(DevSolid) / 4);
+    const uint32_t* gsol = reinterpret_cast<const uint32_t*>(P.solids);
+    uint32_t* lsol = reinterpret_cast<uint32_t*>(s_sol);
+    for (int i = threadIdx.x; i < sw; i += kBlock) lsol[i] = gsol[i];
     for (int i = threadIdx.x; i < P.nbins; i += kBlock) s_eb[i] = P.ebins[i];
+    for (int i = threadIdx.x; i < ne; i += kBlock) {
+      s_spec[i] = P.spectrum[i];
+      s_cdf[i] = P.cdf[i];
+    }
     __syncthreads();
     T.sig = s_sig;
     T.rcp = s_rcp;
+    T.solids = s_sol;
     T.ebins = s_eb;
+    T.spectrum = s_spec;
+    T.cdf = s_cdf;
   }
   Lane L;
   L.r = Rng{0, 0, 0, 0, 0};
@@ -742,9 +773,13 @@ void transport_kernel(RunParams P) {
 
   for (;;) {
     // ---- cold phase: every stopped lane advances until it is back inside
-    //      a phantom or has no work left
-    while (__any(L.mode != kInVoxel && L.mode != kDone)) {
-      if (L.mode != kInVoxel && L.mode != kDone) cold_step<kSingle>(P, T, L, G, c);
+    //      a phantom or has no work left,
+    //      or until enough lanes are stepping again (the rest continue in
+    //      the next navigation phase)
+    for (;;) {
+      const bool cold = L.mode != kInVoxel && L.mode != kDone;
+      if (!__any(cold) || __popcll(__ballot(L.mode == kInVoxel)) >= P.resume_threshold) break;
+      if (cold) cold_step<kSingle>(P, T, L, G, c);
     }
     // ---- hot phase: voxel steps while enough lanes are inside a phantom
     uint64_t act = __ballot(L.mode == kInVoxel);
@@ -755,6 +790,11 @@ void transport_kernel(RunParams P) {
       bool stepped = false;
       if (L.mode == kInVoxel) stepped = hot_step<NP, kTrack>(P, T, kSingle ? P.vox0 : G, L);
       steps += __popcll(__ballot(stepped));
+      // pending interactions resolved in batches, so that scattered photons
+      // keep stepping without waiting for the navigation phase
+      if (__popcll(__ballot(L.mode == kInteract)) >= P.interact_batch) {
+        if (L.mode == kInteract) resolve_interaction(P, T, L, c);
+      }
       act = __ballot(L.mode == kInVoxel);
     } while (__popcll(act) >= need);
   }
@@ -857,7 +897,8 @@ __global__ void primaries_kernel(RunParams P, int n, float* pos, float* dir, flo
   f3 gp, gd;
   float energy;
   int eid;
-  sample_primary(P, P.ebins, r, gp, gd, energy, eid);
+  const Tabs T = {P.tab_sig, P.tab_rcp, P.ebins, P.solids, P.spectrum, P.cdf};
+  sample_primary(P, T, r, gp, gd, energy, eid);
   pos[3 * s] = gp.x; pos[3 * s + 1] = gp.y; pos[3 * s + 2] = gp.z;
   dir[3 * s] = gd.x; dir[3 * s + 1] = gd.y; dir[3 * s + 2] = gd.z;
   e[s] = energy;
@@ -976,6 +1017,8 @@ struct ggh_context {
   DevBuf<unsigned long long> stats;
   // launch shape
   int hot_threshold = 32;
+  int resume_threshold = 64;
+  int interact_batch = 16;
   int blocks_per_cu = 0;  // 0: as many as fit
   bool force_general = false;
   unsigned last_grid = 0;
@@ -1072,6 +1115,8 @@ RunParams make_params(ggh_context* c) {
   P.queue = c->queue.p;
   P.stats = c->stats.p;
   P.hot_threshold = c->hot_threshold;
+  P.resume_threshold = c->resume_threshold;
+  P.interact_batch = c->interact_batch;
   for (const DevSolid& s : c->solids)
     if (s.type == kVoxel) {
       P.vox0 = s.vox;
@@ -1090,8 +1135,12 @@ uint32_t urandom_seed() {
   return s;
 }
 
+// bytes of the LDS image the transport kernel stages per block: step tables,
+// solids, energy bins, spectrum and cdf (n_energy_arg + 1 entries each, as
+// ggh_set_xray_source stores them)
 size_t tables_lds_bytes(const ggh_context* c) {
-  return (size_t)2 * c->n_mat * c->nbins * sizeof(float4) + (size_t)c->nbins * sizeof(float);
+  return (size_t)2 * c->n_mat * c->nbins * sizeof(float4) + c->solids.size() * sizeof(DevSolid) +
+         (size_t)c->nbins * sizeof(float) + (size_t)2 * (c->n_energy_arg + 1) * sizeof(float);
 }
 
 }  // namespace
@@ -1357,14 +1406,17 @@ int ggh_reset_tallies(ggh_context* c) {
   return 0;
 }
 
-int ggh_set_launch(ggh_context* c, int32_t hot_threshold, int32_t blocks_per_cu, int32_t general_kernel) {
+int ggh_set_launch(ggh_context* c, const ggh_launch_desc* d) {
   int rc = check_ctx(c);
   if (rc) return rc;
-  if (hot_threshold < 0 || hot_threshold > 64 || blocks_per_cu < 0 || blocks_per_cu > 64)
+  if (!d || d->hot_threshold < 0 || d->hot_threshold > 64 || d->interact_batch < 0 || d->interact_batch > 64 ||
+      d->resume_threshold < 0 || d->resume_threshold > 64 || d->blocks_per_cu < 0 || d->blocks_per_cu > 64)
     return fail(-2, "invalid launch parameters");
-  c->hot_threshold = hot_threshold ? hot_threshold : 32;
-  c->blocks_per_cu = blocks_per_cu;
-  c->force_general = general_kernel != 0;
+  c->hot_threshold = d->hot_threshold ? d->hot_threshold : 32;
+  c->resume_threshold = std::max(d->resume_threshold ? d->resume_threshold : 64, c->hot_threshold);
+  c->interact_batch = d->interact_batch ? d->interact_batch : 16;
+  c->blocks_per_cu = d->blocks_per_cu;
+  c->force_general = d->general_kernel != 0;
   return 0;
 }
 

@@ -11,7 +11,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_DIR = os.path.join(HERE, "lib")
+LIB_DIR = os.environ.get("GGEMS_AMD_LIB_DIR") or os.path.join(HERE, "lib")
 ENGINE_LIB = os.path.join(LIB_DIR, "libggems_engine.so")
 
 MAX_SLOTS = 1048576
@@ -59,6 +59,11 @@ class XRaySourceDesc(C.Structure):
                 ("n_energy", C.c_int32), ("energy", C.c_void_p), ("cdf", C.c_void_p), ("is_interp", C.c_int32)]
 
 
+class LaunchDesc(C.Structure):
+    _fields_ = [("hot_threshold", C.c_int32), ("resume_threshold", C.c_int32), ("interact_batch", C.c_int32),
+                ("blocks_per_cu", C.c_int32), ("general_kernel", C.c_int32)]
+
+
 class Stats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("primaries", "voxel_steps", "interactions", "detector_steps",
                                           "detector_interactions", "hops")]
@@ -102,7 +107,7 @@ def library():
             "ggh_host_uniform_from_sum": ([vp, vp, C.c_size_t], None),
             "ggh_host_div_rn": ([vp, vp, vp, C.c_size_t], None),
             "ggh_host_distance_to_aabb": ([vp, vp, vp, vp, C.c_size_t], None),
-            "ggh_set_launch": ([vp, i32, i32, i32], C.c_int),
+            "ggh_set_launch": ([vp, C.POINTER(LaunchDesc)], C.c_int),
             "ggh_last_grid": ([vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)], C.c_int),
             "ggh_debug_neg_log": ([vp, C.c_int, vp, vp, vp], C.c_int),
         }
@@ -343,8 +348,10 @@ class Engine:
         _check(library().ggh_debug_primaries(self.h, n, _ptr(pos), _ptr(dir), _ptr(e)))
         return pos, dir, e
 
-    def set_launch(self, hot_threshold=0, blocks_per_cu=0, general_kernel=False):
-        _check(library().ggh_set_launch(self.h, int(hot_threshold), int(blocks_per_cu), int(bool(general_kernel))))
+    def set_launch(self, hot_threshold=0, interact_batch=0, blocks_per_cu=0, general_kernel=False, resume_threshold=0):
+        d = LaunchDesc(int(hot_threshold), int(resume_threshold), int(interact_batch), int(blocks_per_cu),
+                       int(bool(general_kernel)))
+        _check(library().ggh_set_launch(self.h, C.byref(d)))
 
     def last_grid(self):
         """(blocks, block size, kernel variant: 0 specialised, 1 general) of the last run"""

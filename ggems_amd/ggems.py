@@ -13,7 +13,7 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-HOST_LIB = os.path.join(HERE, "lib", "libggems.so")
+HOST_LIB = os.path.join(os.environ.get("GGEMS_AMD_LIB_DIR") or os.path.join(HERE, "lib"), "libggems.so")
 
 
 class GGEMSError(RuntimeError):

@@ -148,12 +148,20 @@ int ggh_last_kernel_ms(ggh_context* ctx, float* ms);
 int ggh_reset_tallies(ggh_context* ctx);
 int ggh_get_stats(ggh_context* ctx, ggh_stats* out);
 
-/* launch shape of ggh_run (no counterpart in GGEMS; for tuning and tests):
- * a wave keeps stepping voxels while at least hot_threshold of its 64 lanes
- * are inside a phantom (0 = default 32); blocks_per_cu caps the persistent
- * blocks per compute unit (0 = as many as fit); general_kernel != 0 runs
- * the general kernel variant even where the specialised one applies */
-int ggh_set_launch(ggh_context* ctx, int32_t hot_threshold, int32_t blocks_per_cu, int32_t general_kernel);
+/* launch shape of ggh_run (no counterpart in GGEMS; for tuning and tests).
+ * Zero fields keep the defaults. */
+typedef struct {
+  int32_t hot_threshold;   /* a wave keeps stepping voxels while at least this many of its 64 lanes
+                              are inside a phantom (default 32) */
+  int32_t resume_threshold;/* the navigation phase ends once this many lanes step voxels again
+                              (default 64: once every lane is back inside a phantom or done) */
+  int32_t interact_batch;  /* inside the voxel loop, resolve pending Compton/Rayleigh/photoelectric
+                              interactions once this many lanes wait for one (default 16; 64 leaves
+                              them to the navigation phase) */
+  int32_t blocks_per_cu;   /* cap on persistent blocks per compute unit (default: as many as fit) */
+  int32_t general_kernel;  /* nonzero: the general kernel variant even where the specialised one applies */
+} ggh_launch_desc;
+int ggh_set_launch(ggh_context* ctx, const ggh_launch_desc* desc);
 /* grid and kernel variant (0 specialised, 1 general) of the last ggh_run */
 int ggh_last_grid(ggh_context* ctx, int32_t* blocks, int32_t* block_size, int32_t* variant);
 

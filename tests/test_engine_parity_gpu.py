@@ -102,13 +102,14 @@ def test_voxel_walk_bit_exact():
     eng.close()
 
 
-@pytest.mark.parametrize("general", [False, True])
-def test_cfg1_run_matches_oracle(cfg1, general):
-    """The same seeded run through the specialised and the general kernel."""
+@pytest.mark.parametrize("general,interact_batch", [(False, 0), (True, 0), (False, 8), (True, 4)])
+def test_cfg1_run_matches_oracle(cfg1, general, interact_batch):
+    """The same seeded run through the specialised and the general kernel,
+    with interactions resolved in the navigation phase or inside the voxel loop."""
     n = 200_000
     ref = cfg1.run_oracle(n, 777, threads=0)
     eng = cfg1.make_engine()
-    eng.set_launch(general_kernel=general)
+    eng.set_launch(interact_batch=interact_batch, general_kernel=general)
     got = cfg1.run_engine(n, 777, eng)
     assert eng.last_grid()[2] == (1 if general else 0)
     rs, gs = ref["stats"], got["stats"]
