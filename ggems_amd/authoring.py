@@ -132,6 +132,66 @@ def write_spectrum(path, energies_mev, intensities):
     return path
 
 
+def icosphere(radius, subdivisions):
+    """Closed triangle mesh of a sphere: an icosahedron whose faces are split
+    in four `subdivisions` times, vertices pushed onto the sphere.  Returns
+    float64 triangles [20 * 4^subdivisions][3][3], outward counter-clockwise."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+         (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    verts = np.array(v, np.float64)
+    verts /= np.linalg.norm(verts, axis=1, keepdims=True)
+    faces = np.array(f, np.int64)
+    for _ in range(subdivisions):
+        cache = {}
+        vlist = list(verts)
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = (vlist[a] + vlist[b]) * 0.5
+                vlist.append(m / np.linalg.norm(m))
+                cache[key] = len(vlist) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        verts = np.array(vlist)
+        faces = np.array(nf, np.int64)
+    return verts[faces] * radius
+
+
+def write_stl(path, triangles, header=b"ggems_amd mesh"):
+    """Binary STL (80-byte header, uint32 count, per triangle: normal, three
+    vertices, uint16 attribute), the format GGEMSSTLReader::Read reads
+    (src/io/GGEMSSTLReader.cc:71-99)."""
+    tri = np.asarray(triangles, np.float64).reshape(-1, 3, 3)
+    n = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    n /= np.maximum(np.linalg.norm(n, axis=1, keepdims=True), 1e-30)
+    rec = np.zeros(len(tri), dtype=[("n", "<f4", 3), ("v", "<f4", (3, 3)), ("a", "<u2")])
+    rec["n"] = n
+    rec["v"] = tri
+    with open(path, "wb") as fh:
+        fh.write(header[:80].ljust(80, b"\0"))
+        fh.write(np.uint32(len(tri)).tobytes())
+        fh.write(rec.tobytes())
+    return path
+
+
+def read_stl(path):
+    """float32 triangles [n][3][3] of a binary STL file"""
+    with open(path, "rb") as fh:
+        fh.read(80)
+        n = int(np.frombuffer(fh.read(4), "<u4")[0])
+        rec = np.frombuffer(fh.read(50 * n), dtype=[("n", "<f4", 3), ("v", "<f4", (3, 3)), ("a", "<u2")])
+    return np.array(rec["v"], np.float32)
+
+
 MATERIALS_TXT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "_gen", "materials.txt")
 
 

@@ -53,7 +53,7 @@ constexpr int kLdsBudget = 64 * 1024;  // step tables + energy bins staged per b
 // ---------------------------------------------------------------------------
 // device-side scene description
 // ---------------------------------------------------------------------------
-enum SolidType : int32_t { kVoxel = 0, kBoxSolid = 1 };
+enum SolidType : int32_t { kVoxel = 0, kBoxSolid = 1, kMesh = 2 };
 
 // Everything the voxel step reads about its phantom.  rvs = RN(1/voxel_size)
 // (exact division, see div_rn); in_lo/in_hi = border -/+ the geometry
@@ -78,6 +78,12 @@ struct DevSolid {
   float threshold;
   int32_t* histogram;
   int32_t* scatter;
+  // meshed solid: BVH nodes (two float4 each: {lo, skip}, {hi, first << 4 | count})
+  // and triangles (three float4 each: v0, e1 = v1 - v0, e2 = v2 - v0)
+  const float4* bvh;
+  const float4* tris;
+  int32_t n_bvh;
+  int32_t pad_[3];
 };
 
 static_assert(sizeof(DevSolid) % 16 == 0, "solids are staged in LDS after 16-byte table entries");
@@ -526,6 +532,150 @@ __device__ void track_box(const RunParams& P, const Tabs& T, const DevSolid& s, 
 }
 
 // ---------------------------------------------------------------------------
+// meshed solid transport (src/kernels/TrackThroughGGEMSMeshSolid.cl)
+// ---------------------------------------------------------------------------
+// MollerTrumboreRayTriangle (TrackThroughGGEMSMeshSolid.cl:119-176), with the
+// edges precomputed on the host (the same float subtractions); dot products
+// accumulate left to right as DotMesh does.
+__device__ inline bool moller_trumbore(f3 pi, f3 d, const float4& v0, const float4& e1, const float4& e2, f3& po) {
+  const f3 h = mk((d.y * e2.z) - (d.z * e2.y), (d.z * e2.x) - (d.x * e2.z), (d.x * e2.y) - (d.y * e2.x));
+  float a = e1.x * h.x;
+  a += e1.y * h.y;
+  a += e1.z * h.z;
+  if (a > -kEps6 && a < kEps6) return false;
+  const float f = 1.0f / a;
+  const f3 sv = mk(pi.x - v0.x, pi.y - v0.y, pi.z - v0.z);
+  float sh = sv.x * h.x;
+  sh += sv.y * h.y;
+  sh += sv.z * h.z;
+  const float u = f * sh;
+  if (u < 0.0f || u > 1.0f) return false;
+  const f3 q = mk((sv.y * e1.z) - (sv.z * e1.y), (sv.z * e1.x) - (sv.x * e1.z), (sv.x * e1.y) - (sv.y * e1.x));
+  float qd = q.x * d.x;
+  qd += q.y * d.y;
+  qd += q.z * d.z;
+  const float v = f * qd;
+  if (v < 0.0f || u + v > 1.0f) return false;
+  float qe = q.x * e2.x;
+  qe += q.y * e2.y;
+  qe += q.z * e2.z;
+  const float t = f * qe;
+  if (!(t > kEps6)) return false;
+  po = mk(pi.x + d.x * t, pi.y + d.y * t, pi.z + d.z * t);
+  return true;
+}
+
+// GetMeshIOPositions (TrackThroughGGEMSMeshSolid.cl:253-314): every
+// intersection of the ray with the mesh, counted, and the two nearest in the
+// order of the reference's stable insertion sort.  The reference scans all
+// octree nodes; here a BVH in depth-first order with skip links is walked
+// without a stack.  Its boxes are widened on the host so that pruning never
+// drops a triangle the exact test would hit.
+__device__ inline int mesh_hits(const DevSolid& s, f3 p, f3 d, f3& q0, f3& q1) {
+  const f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  int n = 0;
+  float d0 = 0.0f, d1 = 0.0f;
+  int i = 0;
+  while (i < s.n_bvh) {
+    const float4 a = s.bvh[2 * i], b = s.bvh[2 * i + 1];
+    float t1 = (a.x - p.x) * inv.x, t2 = (b.x - p.x) * inv.x;
+    float tmin = fmaxf(0.0f, fminf(t1, t2)), tmax = fmaxf(t1, t2);
+    t1 = (a.y - p.y) * inv.y;
+    t2 = (b.y - p.y) * inv.y;
+    tmin = fmaxf(tmin, fminf(t1, t2));
+    tmax = fminf(tmax, fmaxf(t1, t2));
+    t1 = (a.z - p.z) * inv.z;
+    t2 = (b.z - p.z) * inv.z;
+    tmin = fmaxf(tmin, fminf(t1, t2));
+    tmax = fminf(tmax, fmaxf(t1, t2));
+    if (tmin <= tmax) {
+      const int info = __float_as_int(b.w);
+      const int count = info & 15, first = info >> 4;
+      for (int k = 0; k < count; ++k) {
+        const int t = first + k;
+        f3 q;
+        if (!moller_trumbore(p, d, s.tris[3 * t], s.tris[3 * t + 1], s.tris[3 * t + 2], q)) continue;
+        // DistanceSquare (:101-104)
+        const float d2 = (q.x - p.x) * (q.x - p.x) + (q.y - p.y) * (q.y - p.y) + (q.z - p.z) * (q.z - p.z);
+        ++n;
+        if (n == 1) {
+          q0 = q;
+          d0 = d2;
+        } else if (d2 < d0) {
+          q1 = q0;
+          d1 = d0;
+          q0 = q;
+          d0 = d2;
+        } else if (n == 2 || d2 < d1) {
+          q1 = q;
+          d1 = d2;
+        }
+      }
+      ++i;
+    } else {
+      i = __float_as_int(a.w);
+    }
+  }
+  return n;
+}
+
+// track_through_ggems_meshed_solid (TrackThroughGGEMSMeshSolid.cl:322-534):
+// one material (the navigator's first), global coordinates.  A photon
+// outside the mesh is moved onto its first surface; when no surface is
+// ahead it is projected out of the octree box.
+__device__ void track_mesh(const RunParams& P, const Tabs& T, const DevSolid& s, Lane& L) {
+  const int gmat = s.mat_offset;
+  f3 lp = L.p, ld = L.ray.d;
+  bool alive = true;
+  uint32_t steps = 0, inter = 0;
+  for (;;) {
+    f3 q0 = lp, q1 = lp;
+    const int n = mesh_hits(s, lp, ld, q0, q1);
+    if (n == 0) break;
+    f3 next_tri = q0;
+    if ((n & 1) == 0) {
+      lp = q0;
+      next_tri = q1;
+    }
+    float lg[3];
+    draw_logs<-1>(P, L.r, lg);
+    int proc;
+    const int ti = gmat * P.nbins + L.eid;
+    float next = next_interaction<-1>(P, T.sig[ti], T.rcp[ti], lg, proc);
+    ++steps;
+    const float dx = (next_tri.x - lp.x), dy = (next_tri.y - lp.y), dz = (next_tri.z - lp.z);
+    const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+    if (dist <= next) {
+      next = dist + kTol;
+      proc = kTransportation;
+    }
+    lp = add(lp, scl(ld, next));
+    if (proc != kTransportation) {
+      ++inter;
+      const float e0 = L.e;
+      f3 dir = ld;
+      if (!discrete(P, T.ebins, proc, gmat, L.e, L.eid, dir, L.r)) alive = false;
+      if (proc == kCompton || proc == kRayleigh) L.scatter = 1;
+      if (proc != kPhotoelectric) {
+        ld = dir;
+        if (L.e != e0) L.eid = binary_search(L.e, T.ebins, P.nbins);
+      }
+    }
+    if (L.e <= T.sig[gmat * P.nbins].w) alive = false;  // photon_energy_cut_[0]
+    if (!alive) break;
+  }
+  // ComputeDistanceToOBB, then the projection out of the box
+  Ray r;
+  r.set(g2l_dir(s.m, ld));
+  const float dist = distance_to_aabb(g2l_pos(s.m, lp), r, s.border);
+  L.p = add(lp, scl(ld, dist + kTol));
+  L.ray.d = ld;
+  L.mode = alive ? kHop : kNeedPrimary;
+  atomicAdd(&P.stats[6], (unsigned long long)steps);
+  atomicAdd(&P.stats[7], (unsigned long long)inter);
+}
+
+// ---------------------------------------------------------------------------
 // source (src/kernels/GetPrimariesGGEMSXRaySource.cl:52-143)
 // ---------------------------------------------------------------------------
 __device__ void sample_primary(const RunParams& P, const Tabs& T, Rng& r, f3& pos, f3& dir, float& energy,
@@ -706,6 +856,8 @@ __device__ inline void cold_step(const RunParams& P, const Tabs& T, Lane& L, Vox
       L.ray.set(g2l_dir(s.m, L.ray.d));
       if (!kSingle) G = s.vox;
       L.mode = kInVoxel;
+    } else if (s.type == kMesh) {
+      track_mesh(P, T, s, L);
     } else {
       track_box(P, T, s, L, c);
     }
@@ -889,6 +1041,19 @@ __global__ void walk_kernel(RunParams P, int solid, int n_rays, const float* pos
   count[i] = n;
 }
 
+__global__ void mesh_hits_kernel(RunParams P, int solid, int n_rays, const float* pos, const float* dir,
+                                 int32_t* count, float* nearest) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_rays) return;
+  const f3 p = mk(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+  const f3 d = mk(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]);
+  f3 q0 = mk(0.f, 0.f, 0.f), q1 = q0;
+  count[i] = mesh_hits(P.solids[solid], p, d, q0, q1);
+  float* o = nearest + 6 * (size_t)i;
+  o[0] = q0.x; o[1] = q0.y; o[2] = q0.z;
+  o[3] = q1.x; o[4] = q1.y; o[5] = q1.z;
+}
+
 __global__ void primaries_kernel(RunParams P, int n, float* pos, float* dir, float* e) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
@@ -997,6 +1162,7 @@ struct ggh_context {
   std::vector<DevSolid> solids;
   std::vector<DevBuf<uint8_t>> labels;
   std::vector<DevBuf<int32_t>> hist, scat;
+  std::vector<DevBuf<float4>> mesh_bvh, mesh_tris;
   std::vector<int> hist_elems;
   std::vector<int> max_material;  // largest global material index a solid can reach
   DevBuf<DevSolid> solids_dev;
@@ -1194,6 +1360,8 @@ int ggh_context_destroy(ggh_context* c) {
   for (auto& b : c->labels) b.release();
   for (auto& b : c->hist) b.release();
   for (auto& b : c->scat) b.release();
+  for (auto& b : c->mesh_bvh) b.release();
+  for (auto& b : c->mesh_tris) b.release();
   c->solids_dev.release();
   c->edep.release(); c->edep2.release(); c->hit.release(); c->ptrack.release();
   c->spectrum.release(); c->cdf.release(); c->rng.release(); c->queue.release(); c->stats.release();
@@ -1286,6 +1454,8 @@ int ggh_add_voxel_solid(ggh_context* c, const ggh_voxel_solid_desc* d) {
   c->labels.push_back(lab);
   c->hist.push_back(DevBuf<int32_t>());
   c->scat.push_back(DevBuf<int32_t>());
+  c->mesh_bvh.push_back(DevBuf<float4>());
+  c->mesh_tris.push_back(DevBuf<float4>());
   c->hist_elems.push_back(0);
   c->solids.push_back(s);
   c->max_material.push_back(d->material_offset + max_label);
@@ -1322,11 +1492,156 @@ int ggh_add_box_solid(ggh_context* c, const ggh_box_solid_desc* d) {
   c->labels.push_back(DevBuf<uint8_t>());
   c->hist.push_back(h);
   c->scat.push_back(sc);
+  c->mesh_bvh.push_back(DevBuf<float4>());
+  c->mesh_tris.push_back(DevBuf<float4>());
   c->hist_elems.push_back(ne);
   c->solids.push_back(s);
   c->max_material.push_back(d->material_offset);
   c->solids_dirty = true;
   return (int)c->solids.size() - 1;
+}
+
+}  // extern "C"
+
+namespace {
+
+// BVH over a triangle mesh, emitted depth first: a node is followed by its
+// first child; `skip` is the index after its subtree.  Leaves hold up to 4
+// consecutive triangles.  Boxes are widened by 1e-3 mm plus 1e-5 of their
+// coordinates so that rounding in the device's slab test cannot prune a box
+// whose triangles the exact intersection test would hit.
+struct BvhBuild {
+  const float* tri;
+  std::vector<int> order;
+  std::vector<float4> nodes;
+  void bounds(int b, int e, float lo[3], float hi[3]) const {
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = INFINITY;
+      hi[a] = -INFINITY;
+    }
+    for (int k = b; k < e; ++k)
+      for (int v = 0; v < 3; ++v)
+        for (int a = 0; a < 3; ++a) {
+          float x = tri[9 * order[k] + 3 * v + a];
+          lo[a] = std::min(lo[a], x);
+          hi[a] = std::max(hi[a], x);
+        }
+  }
+  float centroid(int t, int a) const { return tri[9 * t + a] + tri[9 * t + 3 + a] + tri[9 * t + 6 + a]; }
+  int build(int b, int e) {
+    const int me = (int)nodes.size() / 2;
+    nodes.push_back(make_float4(0, 0, 0, 0));
+    nodes.push_back(make_float4(0, 0, 0, 0));
+    float lo[3], hi[3];
+    bounds(b, e, lo, hi);
+    for (int a = 0; a < 3; ++a) {
+      const float w = 1e-3f + 1e-5f * std::max(std::fabs(lo[a]), std::fabs(hi[a]));
+      lo[a] -= w;
+      hi[a] += w;
+    }
+    int info = 0;
+    if (e - b <= 4) {
+      info = (b << 4) | (e - b);
+    } else {
+      float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      for (int k = b; k < e; ++k)
+        for (int a = 0; a < 3; ++a) {
+          clo[a] = std::min(clo[a], centroid(order[k], a));
+          chi[a] = std::max(chi[a], centroid(order[k], a));
+        }
+      int axis = 0;
+      for (int a = 1; a < 3; ++a)
+        if (chi[a] - clo[a] > chi[axis] - clo[axis]) axis = a;
+      const int m = (b + e) / 2;
+      std::nth_element(order.begin() + b, order.begin() + m, order.begin() + e,
+                       [&](int x, int y) { return centroid(x, axis) < centroid(y, axis); });
+      build(b, m);
+      build(m, e);
+    }
+    const int skip = (int)nodes.size() / 2;
+    float sk, in;
+    std::memcpy(&sk, &skip, 4);
+    std::memcpy(&in, &info, 4);
+    nodes[2 * me] = make_float4(lo[0], lo[1], lo[2], sk);
+    nodes[2 * me + 1] = make_float4(hi[0], hi[1], hi[2], in);
+    return me;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int ggh_add_mesh_solid(ggh_context* c, const ggh_mesh_solid_desc* d) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!d || !d->triangles || d->n_triangles <= 0 || d->n_triangles >= (1 << 26))
+    return fail(-2, "invalid mesh solid description");
+  if (c->solids.size() >= GGH_MAX_SOLIDS) return fail(-2, "too many solids");
+  const int n = d->n_triangles;
+  BvhBuild B;
+  B.tri = d->triangles;
+  B.order.resize(n);
+  for (int i = 0; i < n; ++i) B.order[i] = i;
+  B.build(0, n);
+  std::vector<float4> tris((size_t)3 * n);
+  for (int k = 0; k < n; ++k) {
+    const float* t = d->triangles + 9 * (size_t)B.order[k];
+    tris[3 * k] = make_float4(t[0], t[1], t[2], 0.f);
+    tris[3 * k + 1] = make_float4(t[3] - t[0], t[4] - t[1], t[5] - t[2], 0.f);
+    tris[3 * k + 2] = make_float4(t[6] - t[0], t[7] - t[1], t[8] - t[2], 0.f);
+  }
+  DevBuf<float4> bvh, tb;
+  if ((rc = dev_upload(bvh, B.nodes.data(), B.nodes.size(), c->stream))) return rc;
+  if ((rc = dev_upload(tb, tris.data(), tris.size(), c->stream))) {
+    bvh.release();
+    return rc;
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  DevSolid s;
+  std::memset(&s, 0, sizeof s);
+  s.m = to_mat34(d->obb.matrix);
+  s.border.lo = mk(d->obb.border_min[0], d->obb.border_min[1], d->obb.border_min[2]);
+  s.border.hi = mk(d->obb.border_max[0], d->obb.border_max[1], d->obb.border_max[2]);
+  s.type = kMesh;
+  s.mat_offset = d->material_offset;
+  s.bvh = bvh.p;
+  s.tris = tb.p;
+  s.n_bvh = (int32_t)(B.nodes.size() / 2);
+  c->labels.push_back(DevBuf<uint8_t>());
+  c->hist.push_back(DevBuf<int32_t>());
+  c->scat.push_back(DevBuf<int32_t>());
+  c->mesh_bvh.push_back(bvh);
+  c->mesh_tris.push_back(tb);
+  c->hist_elems.push_back(0);
+  c->solids.push_back(s);
+  c->max_material.push_back(d->material_offset);
+  c->solids_dirty = true;
+  return (int)c->solids.size() - 1;
+}
+
+int ggh_debug_mesh_hits(ggh_context* c, int solid, int n_rays, const float* pos, const float* dir, int32_t* count,
+                        float* nearest) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (solid < 0 || solid >= (int)c->solids.size() || c->solids[solid].type != kMesh)
+    return fail(-2, "not a meshed solid");
+  if (n_rays <= 0) return fail(-2, "invalid ray count");
+  if ((rc = sync_solids(c))) return rc;
+  RunParams P = make_params(c);
+  DevBuf<float> dp, dd, dn;
+  DevBuf<int32_t> dc;
+  if ((rc = dev_upload(dp, pos, (size_t)3 * n_rays, c->stream)) || (rc = dev_upload(dd, dir, (size_t)3 * n_rays, c->stream)) ||
+      (rc = dev_alloc(dc, (size_t)n_rays)) || (rc = dev_alloc(dn, (size_t)6 * n_rays)))
+    return rc;
+  hipLaunchKernelGGL(mesh_hits_kernel, dim3((n_rays + 63) / 64), dim3(64), 0, c->stream, P, solid, n_rays, dp.p, dd.p,
+                     dc.p, dn.p);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(count, dc.p, (size_t)n_rays * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(nearest, dn.p, (size_t)6 * n_rays * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  dp.release(); dd.release(); dn.release(); dc.release();
+  return 0;
 }
 
 int ggh_set_dosimetry(ggh_context* c, const ggh_dosimetry_desc* d) {
@@ -1499,6 +1814,8 @@ int ggh_get_stats(ggh_context* c, ggh_stats* out) {
   out->detector_steps = h[3];
   out->detector_interactions = h[4];
   out->hops = h[5];
+  out->mesh_steps = h[6];
+  out->mesh_interactions = h[7];
   return 0;
 }
 

@@ -358,6 +358,31 @@ void set_material_color_ggems_voxelized_phantom(GGEMSVoxelizedPhantom*, char con
                                                 unsigned char) {}
 void set_material_color_name_ggems_voxelized_phantom(GGEMSVoxelizedPhantom*, char const*, char const*) {}
 
+// ---- meshed phantom ---------------------------------------------------------------------------
+GGEMSMeshedPhantom* create_ggems_meshed_phantom(char const* name) {
+  return new (std::nothrow) GGEMSMeshedPhantom(str(name));
+}
+void set_phantom_file_ggems_meshed_phantom(GGEMSMeshedPhantom* p, char const* f) { p->phantom_file = str(f); }
+void set_position_ggems_meshed_phantom(GGEMSMeshedPhantom* p, float x, float y, float z, char const* u) {
+  API_TRY p->set_position(x, y, z, str(u));
+  API_CATCH()
+}
+void set_rotation_ggems_meshed_phantom(GGEMSMeshedPhantom* p, float x, float y, float z, char const* u) {
+  API_TRY p->set_rotation(x, y, z, str(u));
+  API_CATCH()
+}
+void set_material_name_ggems_meshed_phantom(GGEMSMeshedPhantom* p, char const* material) {
+  // GGEMSNavigator::SetMaterialName (src/navigators/GGEMSNavigator.cc:258-261)
+  API_TRY p->materials.add(str(material));
+  API_CATCH()
+}
+void set_mesh_octree_depth_ggems_meshed_phantom(GGEMSMeshedPhantom* p, int depth) { p->octree_depth = depth; }
+// OpenGL display settings: accepted, no display in this build
+void set_visible_ggems_meshed_phantom(GGEMSMeshedPhantom*, bool) {}
+void set_material_color_ggems_meshed_phantom(GGEMSMeshedPhantom*, char const*, unsigned char, unsigned char,
+                                             unsigned char) {}
+void set_material_color_name_ggems_meshed_phantom(GGEMSMeshedPhantom*, char const*, char const*) {}
+
 // ---- CT system -----------------------------------------------------------------------------------
 GGEMSCTSystem* create_ggems_ct_system(char const* name) { return new (std::nothrow) GGEMSCTSystem(str(name)); }
 void set_number_of_modules_ggems_ct_system(GGEMSCTSystem* c, size_t x, size_t y) {

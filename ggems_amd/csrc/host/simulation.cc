@@ -2,6 +2,7 @@
 #include "simulation.hh"
 
 #include <algorithm>
+#include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -258,6 +259,96 @@ int GGEMSVoxelizedPhantom::register_solids(ggh_context* ctx, int material_offset
 
 void GGEMSVoxelizedPhantom::save_results(ggh_context* ctx) {
   if (dosimetry) dosimetry->save(ctx);
+}
+
+// ---- GGEMSMeshedPhantom -------------------------------------------------------------------------
+namespace {
+
+// GGEMSSTLReader::Read (src/io/GGEMSSTLReader.cc:71-99): binary STL, the
+// three vertices of each record (the normal and attribute are skipped)
+std::vector<float> read_stl(const std::string& path) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) raise("GGEMSSTLReader", "Problem reading filename: " + path);
+  char header[80];
+  uint32_t n = 0;
+  f.read(header, 80);
+  f.read(reinterpret_cast<char*>(&n), 4);
+  if (!f || n == 0) raise("GGEMSSTLReader", "Invalid STL file: " + path);
+  std::vector<float> tri((size_t)n * 9);
+  for (uint32_t i = 0; i < n; ++i) {
+    float data[12];
+    uint16_t attr;
+    f.read(reinterpret_cast<char*>(data), sizeof data);
+    f.read(reinterpret_cast<char*>(&attr), sizeof attr);
+    if (!f) raise("GGEMSSTLReader", "Truncated STL file: " + path);
+    for (int k = 0; k < 9; ++k) tri[(size_t)9 * i + k] = data[3 + k];
+  }
+  return tri;
+}
+
+}  // namespace
+
+void GGEMSMeshedPhantom::initialize() {
+  // GGEMSMeshedPhantom::CheckParameters / Initialize (src/navigators/GGEMSMeshedPhantom.cc:64-155)
+  if (phantom_file.empty()) raise("GGEMSMeshedPhantom", "You have to set a mesh file in STL format!!!");
+  if (octree_depth > 8) raise("GGEMSOctree", "Max. depth too large in Octree!!!");
+  if (octree_depth < 1) raise("GGEMSOctree", "Octree depth has to be at least 1!!!");
+  if (materials.names.empty()) raise("GGEMSMeshedPhantom", "A material has to be set for the meshed phantom!!!");
+  triangles = read_stl(phantom_file);
+  // rotation before position; UpdateTriangles applies the rows of the
+  // matrix (dot product, left to right) plus its translation to every point
+  // (src/geometries/GGEMSMeshedSolid.cc:205-262)
+  static const float axis[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  Transform t;
+  t.set_axis(axis);
+  if (update_rot) t.rotate(rotation[0], rotation[1], rotation[2]);
+  if (update_pos) t.translate(position[0], position[1], position[2]);
+  const float* m = t.m.data();
+  for (size_t k = 0; k < triangles.size(); k += 3) {
+    const float x = triangles[k], y = triangles[k + 1], z = triangles[k + 2];
+    for (int r = 0; r < 3; ++r) {
+      float v = m[4 * r] * x;
+      v += m[4 * r + 1] * y;
+      v += m[4 * r + 2] * z;
+      triangles[k + r] = v + m[4 * r + 3];
+    }
+  }
+  // ComputeOctreeCenter / ComputeHalfWidthCenter (src/geometries/GGEMSMeshedSolid.cc:340-458):
+  // bounds (the maximum starts at FLT_MIN), grown by 2.5 %, rounded outwards to whole mm
+  float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {FLT_MIN, FLT_MIN, FLT_MIN};
+  for (size_t k = 0; k < triangles.size(); k += 3)
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = triangles[k + a] < lo[a] ? triangles[k + a] : lo[a];
+      hi[a] = triangles[k + a] > hi[a] ? triangles[k + a] : hi[a];
+    }
+  for (int a = 0; a < 3; ++a) {
+    const float e = (hi[a] - lo[a]) * 0.025f;
+    lo[a] -= e;
+    hi[a] += e;
+    lo[a] = lo[a] < 0.0f ? std::floor(lo[a]) : std::ceil(lo[a]);
+    hi[a] = hi[a] < 0.0f ? std::floor(hi[a]) : std::ceil(hi[a]);
+    const float center = (hi[a] + lo[a]) * 0.5f, half = (hi[a] - lo[a]) * 0.5f;
+    obb_min[a] = center - half;
+    obb_max[a] = center + half;
+  }
+  materials.initialize();
+  cross_sections.initialize();
+}
+
+int GGEMSMeshedPhantom::register_solids(ggh_context* ctx, int material_offset) {
+  ggh_mesh_solid_desc d;
+  std::memset(&d, 0, sizeof d);
+  d.obb.matrix[0] = d.obb.matrix[5] = d.obb.matrix[10] = d.obb.matrix[15] = 1.0f;
+  for (int i = 0; i < 3; ++i) {
+    d.obb.border_min[i] = obb_min[i];
+    d.obb.border_max[i] = obb_max[i];
+  }
+  d.material_offset = material_offset;
+  d.n_triangles = (int32_t)(triangles.size() / 9);
+  d.triangles = triangles.data();
+  solid_index = ggh_add_mesh_solid(ctx, &d);
+  engine_check(solid_index, "register meshed phantom");
+  return 1;
 }
 
 void GGEMSCTSystem::initialize() {

@@ -82,6 +82,21 @@ struct GGEMSVoxelizedPhantom : GGEMSNavigator {
   void save_results(ggh_context* ctx) override;
 };
 
+// GGEMSMeshedPhantom + GGEMSMeshedSolid (src/navigators/GGEMSMeshedPhantom.cc,
+// src/geometries/GGEMSMeshedSolid.cc): one material, triangles moved to
+// global coordinates, the octree box as OBB
+struct GGEMSMeshedPhantom : GGEMSNavigator {
+  explicit GGEMSMeshedPhantom(const std::string& n) : GGEMSNavigator(n) {}
+  std::string phantom_file;
+  int octree_depth = 1;
+  std::vector<float> triangles;  // [n][9], global
+  float obb_min[3] = {0, 0, 0}, obb_max[3] = {0, 0, 0};
+  int solid_index = -1;
+  void initialize() override;
+  int register_solids(ggh_context* ctx, int material_offset) override;
+  void save_results(ggh_context*) override {}
+};
+
 // GGEMSCTSystem + GGEMSSystem + GGEMSSolidBox (src/navigators/GGEMSCTSystem.cc,
 // GGEMSSystem.cc, src/geometries/GGEMSSolidBox.cc)
 struct GGEMSCTSystem : GGEMSNavigator {

@@ -48,6 +48,11 @@ class BoxSolidDesc(C.Structure):
                 ("box_size", C.c_float * 3), ("threshold", C.c_float), ("store_scatter", C.c_int32)]
 
 
+class MeshSolidDesc(C.Structure):
+    _fields_ = [("obb", OBB), ("material_offset", C.c_int32), ("n_triangles", C.c_int32),
+                ("triangles", C.c_void_p)]
+
+
 class DosimetryDesc(C.Structure):
     _fields_ = [("solid", C.c_int32), ("dosel_size", C.c_float * 3), ("n_dosels", C.c_int32 * 3),
                 ("border_min", C.c_float * 3), ("border_max", C.c_float * 3),
@@ -66,7 +71,7 @@ class LaunchDesc(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("primaries", "voxel_steps", "interactions", "detector_steps",
-                                          "detector_interactions", "hops")]
+                                          "detector_interactions", "hops", "mesh_steps", "mesh_interactions")]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
@@ -110,6 +115,8 @@ def library():
             "ggh_set_launch": ([vp, C.POINTER(LaunchDesc)], C.c_int),
             "ggh_last_grid": ([vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)], C.c_int),
             "ggh_debug_neg_log": ([vp, C.c_int, vp, vp, vp], C.c_int),
+            "ggh_add_mesh_solid": ([vp, C.POINTER(MeshSolidDesc)], C.c_int),
+            "ggh_debug_mesh_hits": ([vp, C.c_int, C.c_int, vp, vp, vp, vp], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -260,6 +267,27 @@ class Engine:
         self.solid_kinds.append("box")
         self.box_elements[k] = (int(n_elements[1]), int(n_elements[0]), bool(store_scatter))
         return k
+
+    def add_mesh_solid(self, triangles, border_min, border_max, material_offset, matrix=None):
+        """triangles [n][3][3] in global coordinates; the OBB is the octree box"""
+        tri = np.ascontiguousarray(triangles, dtype=np.float32).reshape(-1, 9)
+        d = MeshSolidDesc()
+        d.obb = _obb(np.eye(4, dtype=np.float32) if matrix is None else matrix, border_min, border_max)
+        d.material_offset = int(material_offset)
+        d.n_triangles = len(tri)
+        d.triangles = tri.ctypes.data_as(C.c_void_p)
+        k = _check(library().ggh_add_mesh_solid(self.h, C.byref(d)))
+        self.solid_kinds.append("mesh")
+        return k
+
+    def debug_mesh_hits(self, solid, pos, dir):
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        dir = np.ascontiguousarray(dir, np.float32).reshape(-1, 3)
+        n = pos.shape[0]
+        count = np.zeros(n, np.int32)
+        nearest = np.zeros((n, 2, 3), np.float32)
+        _check(library().ggh_debug_mesh_hits(self.h, solid, n, _ptr(pos), _ptr(dir), _ptr(count), _ptr(nearest)))
+        return count, nearest
 
     def set_dosimetry(self, solid, dosel_size, n_dosels, border_min, border_max, edep_squared=True, hit=True,
                       photon_tracking=False):

@@ -290,6 +290,32 @@ class GGEMSVoxelizedPhantom:
         pass
 
 
+class GGEMSMeshedPhantom:
+    def __init__(self, meshed_phantom_name):
+        self.obj = _call("create_ggems_meshed_phantom", VP, [S], _b(meshed_phantom_name))
+
+    def set_phantom(self, phantom_filename):
+        _call("set_phantom_file_ggems_meshed_phantom", None, [VP, S], self.obj, _b(phantom_filename))
+
+    def set_position(self, pos_x, pos_y, pos_z, unit):
+        _call("set_position_ggems_meshed_phantom", None, [VP, F, F, F, S], self.obj, pos_x, pos_y, pos_z, _b(unit))
+
+    def set_rotation(self, rx, ry, rz, unit):
+        _call("set_rotation_ggems_meshed_phantom", None, [VP, F, F, F, S], self.obj, rx, ry, rz, _b(unit))
+
+    def set_visible(self, flag):
+        pass
+
+    def set_material(self, material_name):
+        _call("set_material_name_ggems_meshed_phantom", None, [VP, S], self.obj, _b(material_name))
+
+    def set_mesh_octree_depth(self, depth):
+        _call("set_mesh_octree_depth_ggems_meshed_phantom", None, [VP, I], self.obj, int(depth))
+
+    def set_material_color(self, material_name, red=0, green=0, blue=0, color_name=""):
+        pass
+
+
 class GGEMSCTSystem:
     def __init__(self, ct_system_name):
         self.obj = _call("create_ggems_ct_system", VP, [S], _b(ct_system_name))

@@ -34,6 +34,7 @@ typedef struct GGEMSProfilerManager GGEMSProfilerManager;
 typedef struct GGEMSRAMManager GGEMSRAMManager;
 typedef struct GGEMSVoxelizedPhantom GGEMSVoxelizedPhantom;
 typedef struct GGEMSCTSystem GGEMSCTSystem;
+typedef struct GGEMSMeshedPhantom GGEMSMeshedPhantom;
 typedef struct GGEMSDosimetryCalculator GGEMSDosimetryCalculator;
 typedef struct GGEMSVolumeCreatorManager GGEMSVolumeCreatorManager;
 typedef struct GGEMSBox GGEMSBox;
@@ -162,6 +163,21 @@ void set_material_color_ggems_voxelized_phantom(GGEMSVoxelizedPhantom* voxelized
                                                 unsigned char red, unsigned char green, unsigned char blue);
 void set_material_color_name_ggems_voxelized_phantom(GGEMSVoxelizedPhantom* voxelized_phantom,
                                                      char const* material_name, char const* color_name);
+
+/* ---- include/GGEMS/navigators/GGEMSMeshedPhantom.hh:125-199 ----------------- */
+GGEMSMeshedPhantom* create_ggems_meshed_phantom(char const* meshed_phantom_name);
+void set_phantom_file_ggems_meshed_phantom(GGEMSMeshedPhantom* meshed_phantom, char const* phantom_filename);
+void set_position_ggems_meshed_phantom(GGEMSMeshedPhantom* meshed_phantom, float position_x, float position_y,
+                                       float position_z, char const* unit);
+void set_rotation_ggems_meshed_phantom(GGEMSMeshedPhantom* meshed_phantom, float rx, float ry, float rz,
+                                       char const* unit);
+void set_visible_ggems_meshed_phantom(GGEMSMeshedPhantom* meshed_phantom, bool flag);
+void set_material_name_ggems_meshed_phantom(GGEMSMeshedPhantom* meshed_phantom, char const* material_name);
+void set_material_color_ggems_meshed_phantom(GGEMSMeshedPhantom* meshed_phantom, char const* material_name,
+                                             unsigned char red, unsigned char green, unsigned char blue);
+void set_material_color_name_ggems_meshed_phantom(GGEMSMeshedPhantom* meshed_phantom, char const* material_name,
+                                                  char const* color_name);
+void set_mesh_octree_depth_ggems_meshed_phantom(GGEMSMeshedPhantom* meshed_phantom, int depth);
 
 /* ---- include/GGEMS/navigators/GGEMSCTSystem.hh:141-280 ---------------------- */
 GGEMSCTSystem* create_ggems_ct_system(char const* ct_system_name);

@@ -92,6 +92,16 @@ typedef struct {
   int32_t store_scatter;
 } ggh_box_solid_desc;
 
+/* Meshed solid (GGEMSMeshedSolid): a closed triangle mesh of one material,
+ * in global coordinates (the OBB matrix is the identity and its borders are
+ * the octree box, src/geometries/GGEMSMeshedSolid.cc:296-458). */
+typedef struct {
+  ggh_obb obb;
+  int32_t material_offset;
+  int32_t n_triangles;
+  const float* triangles;        /* host pointer, [n_triangles][3 points][3] */
+} ggh_mesh_solid_desc;
+
 typedef struct {
   int32_t solid;                 /* voxel solid index returned by ggh_add_voxel_solid */
   float dosel_size[3];
@@ -121,6 +131,8 @@ typedef struct {
   uint64_t detector_steps;
   uint64_t detector_interactions;/* discrete processes inside box solids */
   uint64_t hops;
+  uint64_t mesh_steps;           /* iterations of the meshed-solid transport loop */
+  uint64_t mesh_interactions;    /* discrete processes inside meshed solids */
 } ggh_stats;
 
 const char* ggh_last_error(void);
@@ -134,6 +146,8 @@ int ggh_set_physics(ggh_context* ctx, const ggh_physics_desc* desc);
  * visited in the order they are added, as GGEMS registers them */
 int ggh_add_voxel_solid(ggh_context* ctx, const ggh_voxel_solid_desc* desc);
 int ggh_add_box_solid(ggh_context* ctx, const ggh_box_solid_desc* desc);
+/* meshed solid (GGEMSMeshedSolid::Initialize + BuildOctree, src/navigators/GGEMSMeshedPhantom.cc:80-155) */
+int ggh_add_mesh_solid(ggh_context* ctx, const ggh_mesh_solid_desc* desc);
 int ggh_set_dosimetry(ggh_context* ctx, const ggh_dosimetry_desc* desc);
 int ggh_set_xray_source(ggh_context* ctx, const ggh_xray_source_desc* desc);
 
@@ -176,6 +190,10 @@ int ggh_compute_dose(ggh_context* ctx, float scale_factor, int32_t water_referen
  * interaction; pos/dir are global.  out_index[r*max_steps + k], out_count[r]. */
 int ggh_debug_voxel_walk(ggh_context* ctx, int solid, int n_rays, const float* pos, const float* dir,
                          int max_steps, int32_t* out_index, int32_t* out_count);
+/* ray/mesh intersections of meshed solid `solid` (the transport's
+ * GetMeshIOPositions): count[r] and the two nearest points nearest[r][2][3] */
+int ggh_debug_mesh_hits(ggh_context* ctx, int solid, int n_rays, const float* pos, const float* dir,
+                        int32_t* count, float* nearest);
 /* the first primary of each of the first n slots (consumes their RNG) */
 int ggh_debug_primaries(ggh_context* ctx, int n, float* pos, float* dir, float* energy);
 /* n draws from slot 0..n_slots-1 streams: out[s*n + k] */

@@ -79,7 +79,18 @@ typedef struct {
 float ggo_distance_to_aabb(const float* p, const float* d, const float* bmin, const float* bmax);
 
 /* ---- scene -------------------------------------------------------------- */
-enum { GGO_SOLID_VOXEL = 0, GGO_SOLID_BOX = 1 };
+enum { GGO_SOLID_VOXEL = 0, GGO_SOLID_BOX = 1, GGO_SOLID_MESH = 2 };
+
+/* meshed solid (GGEMSMeshedSolid): triangles in global coordinates and the
+ * GGEMSOctree over them, with each node's triangle list as indices */
+typedef struct {
+  int n_triangles;
+  const float* tri;        /* [n_triangles][9]: p0, p1, p2 */
+  int n_nodes;
+  const float* node_box;   /* [n_nodes][6]: centre xyz, half width xyz */
+  const int* node_first;   /* [n_nodes]: first triangle of the node's list, -1 if none */
+  const int* tri_next;     /* [n_triangles]: next triangle in the same list, -1 at the end */
+} ggo_mesh;
 
 typedef struct {
   int type;
@@ -103,6 +114,8 @@ typedef struct {
   float threshold;
   int32_t* histogram;
   int32_t* scatter;  /* may be NULL */
+  /* meshed solid */
+  ggo_mesh mesh;
 } ggo_solid;
 
 typedef struct {
@@ -137,6 +150,8 @@ typedef struct {
   uint64_t detector_steps;
   uint64_t detector_interactions; /* D: discrete processes in box solids */
   uint64_t hops;
+  uint64_t mesh_steps;
+  uint64_t mesh_interactions;
 } ggo_stats;
 
 /* Transports n_particles primaries with the reference's batch/slot structure
@@ -168,6 +183,17 @@ float ggo_sample_rayleigh(const ggo_physics* ph, int gmat, float e, float* dir,
                           uint32_t* rng, int* status);
 /* one source primary: pos/dir/E */
 void ggo_sample_primary(const ggo_source* src, uint32_t* rng, float* pos, float* dir, float* e);
+
+/* GGEMSMeshedSolid::BuildOctree over triangles already moved to global
+ * coordinates (src/geometries/GGEMSMeshedSolid.cc:296-458,
+ * src/geometries/GGEMSOctree.cc:40-316, src/io/GGEMSSTLReader.cc:111-226):
+ * fills node_box [(8^depth-1)/7][6], node_first, tri_next and the octree
+ * box (the OBB borders).  Returns the node count or -1. */
+int ggo_mesh_build(const float* tri, int n_triangles, int depth, float* node_box, int* node_first,
+                   int* tri_next, float* obb_min, float* obb_max);
+/* GetMeshIOPositions (TrackThroughGGEMSMeshSolid.cl:253-314): number of
+ * ray/mesh intersections and the (up to) 16 nearest points, sorted */
+int ggo_mesh_hits(const ggo_mesh* mesh, const float* p, const float* d, float* points /* [16][3] */);
 
 /* compute_dose_ggems_voxelized_solid (ComputeDoseGGEMSVoxelizedSolid.cl:53-121) */
 void ggo_compute_dose(const ggo_solid* phantom, const float* density_by_gmat,

@@ -42,6 +42,17 @@ class OBB(C.Structure):
     _fields_ = [("m", (C.c_float * 4) * 4), ("bmin", C.c_float * 3), ("bmax", C.c_float * 3)]
 
 
+class Mesh(C.Structure):
+    _fields_ = [
+        ("n_triangles", C.c_int),
+        ("tri", C.c_void_p),
+        ("n_nodes", C.c_int),
+        ("node_box", C.c_void_p),
+        ("node_first", C.c_void_p),
+        ("tri_next", C.c_void_p),
+    ]
+
+
 class Solid(C.Structure):
     _fields_ = [
         ("type", C.c_int),
@@ -65,6 +76,7 @@ class Solid(C.Structure):
         ("threshold", C.c_float),
         ("histogram", C.c_void_p),
         ("scatter", C.c_void_p),
+        ("mesh", Mesh),
     ]
 
 
@@ -98,7 +110,7 @@ class Source(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("primaries", "voxel_steps", "interactions", "detector_steps",
-                                          "detector_interactions", "hops")]
+                                          "detector_interactions", "hops", "mesh_steps", "mesh_interactions")]
 
 
 _lib = None
@@ -144,6 +156,11 @@ def lib():
         L.ggo_sample_primary.argtypes = [C.POINTER(Source), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.ggo_compute_dose.argtypes = [C.POINTER(Solid), C.c_void_p, C.c_float, C.c_int, C.c_float, C.c_void_p,
                                        C.c_void_p]
+        L.ggo_mesh_build.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_void_p]
+        L.ggo_mesh_build.restype = C.c_int
+        L.ggo_mesh_hits.argtypes = [C.POINTER(Mesh), C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ggo_mesh_hits.restype = C.c_int
         L.ggo_data.argtypes = [C.c_char_p, C.POINTER(C.c_uint64)]
         L.ggo_data.restype = C.c_void_p
         if L.ggo_load_physics_data(PHYSICS_BLOB.encode()) != 0:
@@ -220,6 +237,39 @@ def kiss_stream(state5, n):
     for i in range(n):
         out[i] = lib().ggo_kiss_uniform(ptr(s))
     return out, s
+
+
+def mesh_build(triangles, depth):
+    """GGEMS octree over triangles [n][3][3] (global coordinates): returns a
+    dict with the arrays ggo_mesh points into and the octree box."""
+    tri = np.ascontiguousarray(triangles, dtype=np.float32).reshape(-1, 9)
+    n_nodes = (8 ** depth - 1) // 7
+    out = dict(tri=tri, node_box=np.zeros((n_nodes, 6), np.float32), node_first=np.zeros(n_nodes, np.int32),
+               tri_next=np.full(len(tri), -1, np.int32), obb_min=np.zeros(3, np.float32),
+               obb_max=np.zeros(3, np.float32))
+    rc = lib().ggo_mesh_build(ptr(tri), len(tri), depth, ptr(out["node_box"]), ptr(out["node_first"]),
+                              ptr(out["tri_next"]), ptr(out["obb_min"]), ptr(out["obb_max"]))
+    if rc != n_nodes:
+        raise ValueError("mesh build failed (%d)" % rc)
+    return out
+
+
+def mesh_struct(m):
+    s = Mesh()
+    s.n_triangles = len(m["tri"])
+    s.tri = ptr(m["tri"])
+    s.n_nodes = len(m["node_first"])
+    s.node_box = ptr(m["node_box"])
+    s.node_first = ptr(m["node_first"])
+    s.tri_next = ptr(m["tri_next"])
+    return s
+
+
+def mesh_hits(mesh_struct_, p, d):
+    pts = np.zeros((16, 3), np.float32)
+    n = lib().ggo_mesh_hits(C.byref(mesh_struct_), ptr(np.asarray(p, np.float32)), ptr(np.asarray(d, np.float32)),
+                            ptr(pts))
+    return n, pts[:n]
 
 
 def batch_layout(n):

@@ -10,6 +10,7 @@
  *   src/kernels/ProjectToGGEMS*.cl:44-129                      (projection)
  *   src/kernels/TrackThroughGGEMSVoxelizedSolid.cl:58-323      (voxel transport)
  *   src/kernels/TrackThroughGGEMSSolidBox.cl:56-261            (CT module transport)
+ *   src/kernels/TrackThroughGGEMSMeshSolid.cl:322-534          (meshed solid transport)
  *   include/GGEMS/navigators/GGEMSPhotonNavigator.hh:57-131    (next interaction)
  *   include/GGEMS/physics/GGEMSComptonScatteringModels.hh:47-132
  *   include/GGEMS/physics/GGEMSPhotoElectricEffectModels.hh:46-53
@@ -591,6 +592,55 @@ static void track_box(const ggo_physics* ph, const ggo_solid* s, photon* g, uint
   g->d = l2g_dir(m, ld);
 }
 
+static inline float dist_obb(const ggo_obb* o, f3 p, f3 d);
+
+/* track_through_ggems_meshed_solid (TrackThroughGGEMSMeshSolid.cl:322-534): a
+ * single material (the navigator's material 0); global coordinates; the
+ * photon is teleported to the first surface when it is outside the mesh,
+ * and projected out of the octree box (the OBB) when no surface is ahead */
+static void track_mesh(const ggo_physics* ph, const ggo_solid* s, photon* g, uint32_t* rng, ggo_stats* st) {
+  const int gmat = s->mat_offset;
+  f3 lp = g->p, ld = g->d;
+  float pts[16 * 3];
+  do {
+    float pa[3] = {lp.x, lp.y, lp.z}, da[3] = {ld.x, ld.y, ld.z};
+    int n = ggo_mesh_hits(&s->mesh, pa, da, pts);
+    if (n == 0) {
+      g->dist = OUT_OF_WORLD;
+      g->solid = -1;
+      break;
+    }
+    f3 next_tri;
+    if (n % 2 == 0) {
+      lp = v3(pts[0], pts[1], pts[2]);
+      next_tri = v3(pts[3], pts[4], pts[5]);
+    } else {
+      next_tri = v3(pts[0], pts[1], pts[2]);
+    }
+    int proc;
+    float next = next_interaction(ph, g, gmat, rng, &proc);
+    if (st) st->mesh_steps++;
+    float dx = (next_tri.x - lp.x), dy = (next_tri.y - lp.y), dz = (next_tri.z - lp.z);
+    float dtri = sqrtf(dx * dx + dy * dy + dz * dz);
+    if (dtri <= next) {
+      next = dtri + TOL;
+      proc = TRANSPORTATION;
+    }
+    lp = add3(lp, scale3(ld, next));
+    if (proc != TRANSPORTATION) {
+      if (st) st->mesh_interactions++;
+      g->d = ld;
+      discrete(ph, g, proc, gmat, rng);
+      if (proc == COMPTON || proc == RAYLEIGH) g->scatter = 1;
+      ld = g->d;
+    }
+    if (g->e <= ph->photon_cut[gmat]) g->status = DEAD;
+  } while (g->status == ALIVE);
+  float dist = dist_obb(&s->obb, lp, ld);
+  g->p = add3(lp, scale3(ld, dist + TOL));
+  g->d = ld;
+}
+
 /* get_primaries_ggems_xray_source (GetPrimariesGGEMSXRaySource.cl:52-143) */
 void ggo_sample_primary(const ggo_source* src, uint32_t* rng, float* pos, float* dir, float* e) {
   double phi = ggo_kiss_uniform(rng);
@@ -677,6 +727,7 @@ static void transport_primary(const ggo_physics* ph, const ggo_solid* solids, in
     g.dist = 0.0f;
     /* TrackThroughSolid */
     if (s->type == GGO_SOLID_VOXEL) track_voxel(ph, s, &g, rng, st, NULL, 0, NULL);
+    else if (s->type == GGO_SOLID_MESH) track_mesh(ph, s, &g, rng, st);
     else track_box(ph, s, &g, rng, st);
     if (g.status == DEAD) break;
   }
@@ -759,6 +810,8 @@ uint64_t ggo_run_slots(const ggo_physics* ph, const ggo_solid* solids, int ns, c
     tot.detector_steps += ws[t].st.detector_steps;
     tot.detector_interactions += ws[t].st.detector_interactions;
     tot.hops += ws[t].st.hops;
+    tot.mesh_steps += ws[t].st.mesh_steps;
+    tot.mesh_interactions += ws[t].st.mesh_interactions;
   }
   if (stats) *stats = tot;
   free(ws);

@@ -29,6 +29,9 @@ _libm.cosf.restype = C.c_float
 _libm.cosf.argtypes = [C.c_float]
 _libm.sinf.restype = C.c_float
 _libm.sinf.argtypes = [C.c_float]
+for _fn in ("sqrtf", "asinf"):
+    getattr(_libm, _fn).restype = C.c_float
+    getattr(_libm, _fn).argtypes = [C.c_float]
 
 F = np.float32
 DEG = F(3.141592653589793238463) / F(180.0)
@@ -116,6 +119,45 @@ class Scene:
         self.navigators.append(("ct", [material], dict(modules=mods, threshold=F(threshold),
                                                         store_scatter=store_scatter, n_modules=n_modules)))
 
+    def add_ct_curved(self, material, n_modules, n_elements, element_size, sid, sdd, threshold, store_scatter=True):
+        """GGEMSCTSystem::InitializeCurvedGeometry (src/navigators/GGEMSCTSystem.cc:131-183): modules
+        turned about the source by alpha = 2 asin(c / rho) per module."""
+        es = np.array(element_size, F)
+        ne = np.array(n_elements, np.int32)
+        size = ne.astype(F) * es
+        c = F(F(ne[1]) * es[1]) * F(0.5)
+        rho = F(_libm.sqrtf(F(F(sdd) * F(sdd) + c * c)))
+        alpha = F(2.0) * F(_libm.asinf(F(c / rho)))
+        ox, oy, px, py = -F(sid), F(0.0), F(sdd) - F(sid), F(0.0)
+        mods = []
+        for j in range(n_modules[1]):
+            step = F(alpha * F(F(j) + F(0.5) * (F(1.0) - F(n_modules[1]))))
+            cs, sn = F(_libm.cosf(step)), F(_libm.sinf(step))
+            gx = F(F(F((px - ox) * cs) - F((py - oy) * sn)) + ox)
+            gy = F(F(F((px - ox) * sn) + F((py - oy) * cs)) + oy)
+            for i in range(n_modules[0]):
+                gz = F(ne[0]) * es[0] * (F(i) + F(0.5) * (F(1.0) - F(n_modules[0])))
+                tr = Transform(AXIS_SOLID_BOX).rotate((0, 0, step)).translate((gx, gy, gz))
+                mods.append(dict(m=tr.m, bmin=-size * F(0.5), bmax=size * F(0.5), ne=ne, size=size))
+        self.navigators.append(("ct", [material], dict(modules=mods, threshold=F(threshold),
+                                                        store_scatter=store_scatter, n_modules=n_modules)))
+
+    def add_mesh(self, triangles, material, position=(0, 0, 0), rotation=(0, 0, 0), depth=4):
+        """GGEMSMeshedPhantom: triangles [n][3][3] (STL frame) moved by the
+        navigator's rotation then position, as GGEMSMeshedSolid::UpdateTriangles
+        does (src/geometries/GGEMSMeshedSolid.cc:205-262), and the GGEMS octree."""
+        tr = Transform(AXIS_IDENTITY).rotate(rotation).translate(position)
+        m = tr.m
+        tri = np.asarray(triangles, F).reshape(-1, 3, 3)
+        out = np.empty_like(tri)
+        for a in range(3):
+            acc = F(m[a, 0]) * tri[:, :, 0]
+            acc = (acc + F(m[a, 1]) * tri[:, :, 1]).astype(F)
+            acc = (acc + F(m[a, 2]) * tri[:, :, 2]).astype(F)
+            out[:, :, a] = (acc + F(m[a, 3])).astype(F)
+        mesh = O.mesh_build(out, depth)
+        self.navigators.append(("mesh", [material], dict(tri=out, mesh=mesh)))
+
     def set_source(self, position, aperture_deg, energies, intensities=None, focal=(0, 0, 0), rotation=(0, 0, 0),
                    mono=False):
         """GGEMSXRaySource + GGEMSSource::FillEnergy (src/sources/GGEMSSource.cc:265-316)."""
@@ -199,6 +241,15 @@ class Scene:
                         tallies["photon_tracking"] = np.zeros(shape, np.int32)
                         s.photon_tracking = O.ptr(tallies["photon_tracking"])
                 solids.append(s)
+            elif kind == "mesh":
+                s = O.Solid()
+                s.type = 2
+                s.obb.m = ((C.c_float * 4) * 4)(*[(C.c_float * 4)(*row) for row in np.eye(4, dtype=F).tolist()])
+                s.obb.bmin[:] = pay["mesh"]["obb_min"].tolist()
+                s.obb.bmax[:] = pay["mesh"]["obb_max"].tolist()
+                s.mat_offset = off
+                s.mesh = O.mesh_struct(pay["mesh"])
+                solids.append(s)
             else:
                 for k, mod in enumerate(pay["modules"]):
                     s = O.Solid()
@@ -274,6 +325,9 @@ class Scene:
                     d = self.dose_grid(pay)
                     eng.set_dosimetry(k, d["size"], d["n"], pay["bmin"], pay["bmax"], True, True,
                                       bool(pay["dosimetry"].get("photon_tracking")))
+            elif kind == "mesh":
+                k = eng.add_mesh_solid(pay["tri"], pay["mesh"]["obb_min"], pay["mesh"]["obb_max"], off)
+                self.engine_solids.setdefault("mesh", []).append(k)
             else:
                 for mod in pay["modules"]:
                     k = eng.add_box_solid(mod["m"], mod["bmin"], mod["bmax"], mod["ne"], mod["size"],

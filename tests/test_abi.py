@@ -37,7 +37,6 @@ NOT_EXPORTED = {
     "GGEMS/graphics/GGEMSOpenGLManager.hh": "OpenGL display, out of scope",
     "GGEMS/navigators/GGEMSWorld.hh": "world-grid photon tracking, outside the transport hot path",
     "GGEMS/sources/GGEMSVoxelizedSource.hh": "voxelized (emission) source, outside the transport hot path",
-    "GGEMS/navigators/GGEMSMeshedPhantom.hh": "mesh navigation, not built yet",
     "GGEMS/geometries/GGEMSVolumeCreatorManager.hh": "volume creator, not built yet",
     "GGEMS/geometries/GGEMSBox.hh": "volume creator, not built yet",
     "GGEMS/geometries/GGEMSTube.hh": "volume creator, not built yet",
@@ -54,7 +53,7 @@ def test_ggems_header_covers_the_reference_exports():
     assert len(rows) == 177
     missing = [(h, n) for h, n in rows if h not in NOT_EXPORTED and n not in ours]
     assert not missing, missing
-    assert sum(1 for h, n in rows if h not in NOT_EXPORTED) >= 109
+    assert sum(1 for h, n in rows if h not in NOT_EXPORTED) >= 118
 
 
 def test_engine_reports_errors_without_a_device():

@@ -1,0 +1,81 @@
+"""Meshed-solid navigation on the GPU against the oracle: the ray/mesh
+intersection list, and transport through nested icosphere meshes with a
+curved CT detector, through the engine and through a GGEMS script."""
+
+import math
+import os
+
+import numpy as np
+import pytest
+
+import ggems_oracle as O
+from ggems_amd import authoring
+from scene import Scene
+
+pytestmark = pytest.mark.gpu
+F = np.float32
+
+
+def _rays(rng, n, r_out, r_in):
+    target = rng.normal(size=(n, 3))
+    target *= (rng.uniform(0, r_in, n) / np.linalg.norm(target, axis=1))[:, None]
+    p = rng.normal(size=(n, 3))
+    p *= (r_out / np.linalg.norm(p, axis=1))[:, None]
+    inside = rng.random(n) < 0.3
+    p[inside] = target[inside] * 0.5
+    d = rng.normal(size=(n, 3))
+    d[~inside] = target[~inside] - p[~inside]
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return p.astype(F), d.astype(F)
+
+
+def test_mesh_hits_match_oracle():
+    sc = Scene()
+    sc.add_mesh(authoring.icosphere(50.0, 4), "Water", rotation=(0.31, 0.17, 0.11), depth=4)
+    sc.add_ct_flat("GOS", (1, 1), (16, 16, 1), (1.0, 1.0, 1.0), 600.0, 1000.0, 10e-3)
+    sc.set_source((-600.0, 0.0, 0.0), 1.0, [0.08], mono=True)
+    sc.build()
+    eng = sc.make_engine()
+    rng = np.random.default_rng(21)
+    p, d = _rays(rng, 20000, 120.0, 45.0)
+    cnt, near = eng.debug_mesh_hits(0, p, d)
+    ms = O.mesh_struct(sc.navigators[0][2]["mesh"])
+    bad = 0
+    for i in range(len(p)):
+        n, pts = O.mesh_hits(ms, p[i], d[i])
+        ok = n == cnt[i]
+        if ok and n >= 1:
+            ok = np.array_equal(pts[0], near[i, 0])
+        if ok and n >= 2:
+            ok = np.array_equal(pts[1], near[i, 1])
+        bad += not ok
+    assert bad == 0
+    assert (cnt == 2).sum() > 10000 and (cnt == 1).sum() > 3000
+    eng.close()
+
+
+def _nested_scene(curved=True):
+    sc = Scene()
+    sc.add_mesh(authoring.icosphere(50.0, 4), "SoftTissus_15", rotation=(0.31, 0.17, 0.11), depth=4)
+    sc.add_mesh(authoring.icosphere(15.0, 3), "RibBone", rotation=(0.11, 0.23, 0.05), depth=3)
+    if curved:
+        sc.add_ct_curved("GOS", (1, 11), (32, 8, 1), (1.0, 1.0, 1.0), 600.0, 1040.0, 10e-3)
+    else:
+        sc.add_ct_flat("GOS", (1, 1), (128, 128, 1), (1.0, 1.0, 1.0), 600.0, 1000.0, 10e-3)
+    sc.set_source((-600.0, 0.0, 0.0), 4.0, [0.08], mono=True)
+    return sc.build()
+
+
+@pytest.mark.parametrize("curved", [False, True])
+def test_mesh_transport_matches_oracle(curved):
+    sc = _nested_scene(curved)
+    n = 200_000
+    ref = sc.run_oracle(n, 31, threads=0)
+    got = sc.run_engine(n, 31)
+    rs, gs = ref["stats"], got["stats"]
+    assert gs["primaries"] == rs["primaries"] == n
+    for k in ("mesh_steps", "mesh_interactions", "hops"):
+        assert abs(gs[k] - rs[k]) <= 0.01 * rs[k] + 10, (k, gs[k], rs[k])
+    h_ref = np.concatenate([h.ravel() for h in ref["hist"]]).astype(np.int64)
+    h_got = np.concatenate([h.ravel() for h in got["hist"]]).astype(np.int64)
+    assert np.abs(h_ref - h_got).sum() <= 0.02 * h_ref.sum(), (np.abs(h_ref - h_got).sum(), h_ref.sum())
