@@ -7,8 +7,12 @@ voxel_ct   the 512x512x256 body of 0.8 mm voxels (air, lung, adipose,
            soft tissue, cortical bone; 40 seeded ellipsoids) with the cfg1
            source/detector geometry, a 120 kVp Kramers spectrum with 2.5 mm Al
            filtration and FP64 dose + uncertainty tallies.
+mesh_ct    two meshed phantoms (an icosphere of radius 50 mm and 81,920
+           triangles in soft tissue, one of radius 15 mm in cortical bone) in
+           a 300 mm water box, an 80 keV point cone beam at SID 600 mm and a
+           curved CT detector of 888 columns x 64 rows on a 1040 mm arc.
 
-Physics in both: Compton, photoelectric, Rayleigh; cross-section tables of
+Physics in all: Compton, photoelectric, Rayleigh; cross-section tables of
 220 bins over 1 keV - 1 MeV; photon range cut 0.1 mm; detector threshold
 10 keV (as examples/2_CT_Scanner/ct_scanner.py of GGEMS).
 """
@@ -18,8 +22,8 @@ import os
 
 from . import authoring
 from .ggems import (GGEMS, GGEMSCTSystem, GGEMSDosimetryCalculator, GGEMSMaterialsDatabaseManager,
-                    GGEMSOpenCLManager, GGEMSProcessesManager, GGEMSRangeCutsManager, GGEMSVerbosity,
-                    GGEMSVoxelizedPhantom, GGEMSXRaySource, clean_safely)
+                    GGEMSMeshedPhantom, GGEMSOpenCLManager, GGEMSProcessesManager, GGEMSRangeCutsManager,
+                    GGEMSVerbosity, GGEMSVoxelizedPhantom, GGEMSXRaySource, clean_safely)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 MATERIALS_TXT = os.path.join(HERE, "data", "_gen", "materials.txt")
@@ -142,6 +146,92 @@ def voxel_ct(workdir, n_particles=100_000_000, seed=20240, dosimetry=True, shape
     _common_physics()
     src = _source(n_particles)
     src.set_polyenergy(spec)
+    keep.append(src)
+    return keep
+
+
+MESH_SID, MESH_SDD = 600.0, 1040.0
+MESH_MODULES, MESH_ELEMENTS = (1, 37), (64, 24, 1)  # 37 x 24 = 888 columns along the arc, 64 rows
+
+
+def mesh_ct_inputs(workdir, outer_subdivisions=6, inner_subdivisions=4, box_mm=300.0, box_voxel=2.0):
+    """Writes the two icosphere STL files and the water-box phantom (cached)."""
+    os.makedirs(workdir, exist_ok=True)
+    outer = os.path.join(workdir, "icosphere_r50_s%d.stl" % outer_subdivisions)
+    inner = os.path.join(workdir, "icosphere_r15_s%d.stl" % inner_subdivisions)
+    n = int(round(box_mm / box_voxel))
+    box = os.path.join(workdir, "water_box_%d_%g.mhd" % (n, box_voxel))
+    rng = os.path.join(workdir, "range_water.txt")
+    if not os.path.exists(outer):
+        authoring.write_stl(outer, authoring.icosphere(50.0, outer_subdivisions))
+    if not os.path.exists(inner):
+        authoring.write_stl(inner, authoring.icosphere(15.0, inner_subdivisions))
+    if not os.path.exists(box):
+        import numpy as np
+        authoring.write_mhd(box, np.zeros((n, n, n), np.uint8), (box_voxel, box_voxel, box_voxel))
+    authoring.write_range(rng, ["Water"])
+    return outer, inner, box, rng
+
+
+def mesh_ct(workdir, n_particles=100_000_000, water_box=True, dosimetry=False, outer_subdivisions=6,
+            inner_subdivisions=4, box_voxel=2.0, aperture_deg=5.0):
+    """The meshed-phantom scene.  The meshes are registered before the water
+    box.  GGEMS gives a particle to the first solid whose OBB holds it and
+    tracks it there until it leaves that OBB, so with water_box=True every
+    photon is claimed by the enclosing box and crosses the meshes as water;
+    water_box=False leaves the meshes in vacuum, where they are navigated."""
+    outer_stl, inner_stl, box_mhd, box_rng = mesh_ct_inputs(workdir, outer_subdivisions, inner_subdivisions,
+                                                            box_voxel=box_voxel)
+    setup(workdir)
+    keep = []
+    for name, stl, material in (("outer_sphere", outer_stl, "SoftTissus_15"), ("inner_sphere", inner_stl, "RibBone")):
+        m = GGEMSMeshedPhantom(name)
+        m.set_phantom(stl)
+        m.set_rotation(0.0, 0.0, 0.0, "deg")
+        m.set_position(0.0, 0.0, 0.0, "mm")
+        m.set_mesh_octree_depth(4)
+        m.set_material(material)
+        keep.append(m)
+    if water_box:
+        box = GGEMSVoxelizedPhantom("water_box")
+        box.set_phantom(box_mhd, box_rng)
+        box.set_rotation(0.0, 0.0, 0.0, "deg")
+        box.set_position(0.0, 0.0, 0.0, "mm")
+        keep.append(box)
+    if dosimetry:
+        dose = GGEMSDosimetryCalculator()
+        dose.attach_to_navigator("outer_sphere")
+        dose.set_output_basename(os.path.join(workdir, "dosimetry"))
+        dose.set_dosel_size(1.0, 1.0, 1.0, "mm")
+        dose.water_reference(False)
+        dose.minimum_density(0.1, "g/cm3")
+        dose.uncertainty(True)
+        dose.edep(True)
+        dose.hit(True)
+        dose.edep_squared(True)
+        keep.append(dose)
+    ct = GGEMSCTSystem("detector")
+    ct.set_ct_type("curved")
+    ct.set_number_of_modules(*MESH_MODULES)
+    ct.set_number_of_detection_elements(*MESH_ELEMENTS)
+    ct.set_size_of_detection_elements(1.0, 1.0, 1.0, "mm")
+    ct.set_material("GOS")
+    ct.set_source_detector_distance(MESH_SDD, "mm")
+    ct.set_source_isocenter_distance(MESH_SID, "mm")
+    ct.set_rotation(0.0, 0.0, 0.0, "deg")
+    ct.set_threshold(10.0, "keV")
+    ct.save(os.path.join(workdir, "projection"))
+    ct.store_scatter(True)
+    keep.append(ct)
+    _common_physics()
+    src = GGEMSXRaySource("point_source")
+    src.set_source_particle_type("gamma")
+    src.set_number_of_particles(n_particles)
+    src.set_position(-MESH_SID, 0.0, 0.0, "mm")
+    src.set_rotation(0.0, 0.0, 0.0, "deg")
+    src.set_beam_aperture(aperture_deg, "deg")
+    src.set_focal_spot_size(0.0, 0.0, 0.0, "mm")
+    src.set_monoenergy(80.0, "keV")
     keep.append(src)
     return keep
 

@@ -625,6 +625,7 @@ __device__ inline int mesh_hits(const DevSolid& s, f3 p, f3 d, f3& q0, f3& q1) {
 // ahead it is projected out of the octree box.
 __device__ void track_mesh(const RunParams& P, const Tabs& T, const DevSolid& s, Lane& L) {
   const int gmat = s.mat_offset;
+  const bool dose = L.solid == P.dose_solid;
   f3 lp = L.p, ld = L.ray.d;
   bool alive = true;
   uint32_t steps = 0, inter = 0;
@@ -648,6 +649,10 @@ __device__ void track_mesh(const RunParams& P, const Tabs& T, const DevSolid& s,
     if (dist <= next) {
       next = dist + kTol;
       proc = kTransportation;
+      if (dose && P.ptrack) {  // dose_photon_tracking at the point of departure
+        const int id = dosel_index(P, lp);
+        if (id >= 0) atomicAdd(&P.ptrack[id], 1);
+      }
     }
     lp = add(lp, scl(ld, next));
     if (proc != kTransportation) {
@@ -656,12 +661,16 @@ __device__ void track_mesh(const RunParams& P, const Tabs& T, const DevSolid& s,
       f3 dir = ld;
       if (!discrete(P, T.ebins, proc, gmat, L.e, L.eid, dir, L.r)) alive = false;
       if (proc == kCompton || proc == kRayleigh) L.scatter = 1;
+      if (dose) dose_record(P, e0 - L.e, lp);
       if (proc != kPhotoelectric) {
         ld = dir;
         if (L.e != e0) L.eid = binary_search(L.e, T.ebins, P.nbins);
       }
     }
-    if (L.e <= T.sig[gmat * P.nbins].w) alive = false;  // photon_energy_cut_[0]
+    if (L.e <= T.sig[gmat * P.nbins].w) {  // photon_energy_cut_[0]
+      if (dose) dose_record(P, L.e, lp);
+      alive = false;
+    }
     if (!alive) break;
   }
   // ComputeDistanceToOBB, then the projection out of the box
@@ -963,7 +972,8 @@ void transport_kernel(RunParams P) {
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// dose finalisation (src/kernels/ComputeDoseGGEMSVoxelizedSolid.cl:53-121)
+// dose finalisation (src/kernels/ComputeDoseGGEMSVoxelizedSolid.cl:53-121, and
+// ComputeDoseGGEMSMeshSolid.cl:53-114 for a meshed solid: its one material)
 // ---------------------------------------------------------------------------
 namespace {
 
@@ -1002,10 +1012,13 @@ __global__ __launch_bounds__(kDoseBlock) void dose_kernel(DoseParams D) {
     float px = (float)x * D.dosel_size.x + (float)(D.nd[0] - 1) * (-0.5f * D.dosel_size.x);
     float py = (float)y * D.dosel_size.y + (float)(D.nd[1] - 1) * (-0.5f * D.dosel_size.y);
     float pz = (float)z * D.dosel_size.z + (float)(D.nd[2] - 1) * (-0.5f * D.dosel_size.z);
-    int vx = min(max((int)((px - D.vlo.x) / D.vs.x), 0), D.nv[0] - 1);
-    int vy = min(max((int)((py - D.vlo.y) / D.vs.y), 0), D.nv[1] - 1);
-    int vz = min(max((int)((pz - D.vlo.z) / D.vs.z), 0), D.nv[2] - 1);
-    int mat = D.labels[vx + vy * D.nv[0] + vz * D.nv[0] * D.nv[1]];
+    int mat = 0;  // a meshed solid (no labels) holds one material
+    if (D.labels) {
+      int vx = min(max((int)((px - D.vlo.x) / D.vs.x), 0), D.nv[0] - 1);
+      int vy = min(max((int)((py - D.vlo.y) / D.vs.y), 0), D.nv[1] - 1);
+      int vz = min(max((int)((pz - D.vlo.z) / D.vs.z), 0), D.nv[2] - 1);
+      mat = D.labels[vx + vy * D.nv[0] + vz * D.nv[0] * D.nv[1]];
+    }
     float density = D.water_reference ? kWater : D.density[D.mat_offset + mat];
     double e = D.edep[gid];
     D.dose[gid] = density < D.min_density ? 0.0f : (float)((double)D.scale * e / (double)density / (double)vol / (double)kGy);
@@ -1647,8 +1660,8 @@ int ggh_debug_mesh_hits(ggh_context* c, int solid, int n_rays, const float* pos,
 int ggh_set_dosimetry(ggh_context* c, const ggh_dosimetry_desc* d) {
   int rc = check_ctx(c);
   if (rc) return rc;
-  if (!d || d->solid < 0 || d->solid >= (int)c->solids.size() || c->solids[d->solid].type != kVoxel)
-    return fail(-2, "dosimetry needs a voxel solid");
+  if (!d || d->solid < 0 || d->solid >= (int)c->solids.size() || c->solids[d->solid].type == kBoxSolid)
+    return fail(-2, "dosimetry needs a voxel or meshed solid");
   size_t n = (size_t)d->n_dosels[0] * d->n_dosels[1] * d->n_dosels[2];
   if (d->n_dosels[0] <= 0 || d->n_dosels[1] <= 0 || d->n_dosels[2] <= 0 || n > (size_t)INT32_MAX)
     return fail(-2, "invalid dosel grid");
@@ -1874,7 +1887,7 @@ int ggh_compute_dose(ggh_context* c, float scale, int32_t water_ref, float min_d
   D.edep = c->edep.p;
   D.edep2 = c->edep2.p;
   D.hit = c->hit.p;
-  D.labels = s.vox.labels;
+  D.labels = s.type == kVoxel ? s.vox.labels : nullptr;
   D.density = c->density.p;
   D.mat_offset = s.mat_offset;
   D.nd[0] = c->dose_desc.n_dosels[0];

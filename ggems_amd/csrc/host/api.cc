@@ -472,10 +472,12 @@ void dose_tle_navigator(GGEMSDosimetryCalculator*, bool f) {
 }
 void attach_to_navigator_dosimetry_calculator(GGEMSDosimetryCalculator* d, char const* name) {
   API_TRY GGEMSNavigator* n = GGEMSNavigatorManager::get().find(str(name));
-  auto* p = dynamic_cast<GGEMSVoxelizedPhantom*>(n);
-  if (!p) raise("GGEMSDosimetryCalculator", "dosimetry attaches to a voxelized phantom");
-  d->navigator = p;
-  p->dosimetry = d;
+  int solid;
+  float vs[3], lo[3], hi[3];
+  if (!n || !n->dose_geometry(solid, vs, lo, hi))
+    raise("GGEMSDosimetryCalculator", "dosimetry attaches to a voxelized or meshed phantom");
+  d->navigator = n;
+  n->dosimetry = d;
   API_CATCH()
 }
 
@@ -511,6 +513,8 @@ void get_engine_stats_ggems(GGEMS* g, uint64_t* out) {
   out[3] = s.detector_steps;
   out[4] = s.detector_interactions;
   out[5] = s.hops;
+  out[6] = s.mesh_steps;
+  out[7] = s.mesh_interactions;
   API_CATCH()
 }
 

@@ -261,6 +261,16 @@ void GGEMSVoxelizedPhantom::save_results(ggh_context* ctx) {
   if (dosimetry) dosimetry->save(ctx);
 }
 
+bool GGEMSVoxelizedPhantom::dose_geometry(int& solid, float vs[3], float lo[3], float hi[3]) const {
+  solid = solid_index;
+  for (int i = 0; i < 3; ++i) {
+    vs[i] = voxel_size[i];
+    lo[i] = border_min[i];
+    hi[i] = border_max[i];
+  }
+  return true;
+}
+
 // ---- GGEMSMeshedPhantom -------------------------------------------------------------------------
 namespace {
 
@@ -349,6 +359,21 @@ int GGEMSMeshedPhantom::register_solids(ggh_context* ctx, int material_offset) {
   solid_index = ggh_add_mesh_solid(ctx, &d);
   engine_check(solid_index, "register meshed phantom");
   return 1;
+}
+
+void GGEMSMeshedPhantom::save_results(ggh_context* ctx) {
+  if (dosimetry) dosimetry->save(ctx);
+}
+
+// GGEMSMeshedSolid::GetVoxelSizes is zero; the dose map spans the octree box
+bool GGEMSMeshedPhantom::dose_geometry(int& solid, float vs[3], float lo[3], float hi[3]) const {
+  solid = solid_index;
+  for (int i = 0; i < 3; ++i) {
+    vs[i] = 0.0f;
+    lo[i] = obb_min[i];
+    hi[i] = obb_max[i];
+  }
+  return true;
 }
 
 void GGEMSCTSystem::initialize() {
@@ -463,25 +488,33 @@ void GGEMSCTSystem::save_results(ggh_context* ctx) {
 // ---------------------------------------------------------------------------
 void GGEMSDosimetryCalculator::initialize(ggh_context* ctx) {
   if (!navigator) raise("GGEMSDosimetryCalculator", "A navigator has to be associated to GGEMSDosimetryCalculator!!!");
+  int solid = -1;
+  float nav_vs[3], lo[3], hi[3];
+  if (!navigator->dose_geometry(solid, nav_vs, lo, hi))
+    raise("GGEMSDosimetryCalculator", "dosimetry attaches to a voxelized or meshed phantom");
   float vs[3] = {dosel_size[0], dosel_size[1], dosel_size[2]};
   if (dosel_size[0] < 0.0f && dosel_size[1] < 0.0f && dosel_size[2] < 0.0f)
-    for (int i = 0; i < 3; ++i) vs[i] = navigator->voxel_size[i];
+    for (int i = 0; i < 3; ++i) vs[i] = nav_vs[i];
   if (photon_tracking)
     for (int i = 0; i < 3; ++i)
-      if (vs[i] != navigator->voxel_size[i])
+      if (vs[i] != nav_vs[i])
         raise("GGEMSDosimetryCalculator",
               "Dosel size and voxel size in voxelized phantom have to be the same when photon tracking is activated!!!");
+  // a meshed solid has no voxel size: without a dosel size its dose map
+  // would have a zero-sized dosel
+  for (int i = 0; i < 3; ++i)
+    if (!(vs[i] > 0.0f)) raise("GGEMSDosimetryCalculator", "Dosel size has to be set for a meshed phantom!!!");
   ggh_dosimetry_desc d;
   std::memset(&d, 0, sizeof d);
-  d.solid = navigator->solid_index;
+  d.solid = solid;
   for (int i = 0; i < 3; ++i) {
     used_size[i] = vs[i];
     d.dosel_size[i] = vs[i];
-    float extent = navigator->border_max[i] - navigator->border_min[i];
+    float extent = hi[i] - lo[i];
     n_dosels[i] = (int)static_cast<size_t>(extent / vs[i]);
     d.n_dosels[i] = n_dosels[i];
-    d.border_min[i] = navigator->border_min[i];
-    d.border_max[i] = navigator->border_max[i];
+    d.border_min[i] = lo[i];
+    d.border_max[i] = hi[i];
   }
   d.edep_squared = (edep_squared || uncertainty) ? 1 : 0;
   d.hit = (hit || uncertainty) ? 1 : 0;

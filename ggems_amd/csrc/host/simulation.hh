@@ -64,6 +64,10 @@ struct GGEMSNavigator {
   virtual void initialize() = 0;                 // geometry + tables
   virtual int register_solids(ggh_context* ctx, int material_offset) = 0;  // returns solids added
   virtual void save_results(ggh_context* ctx) = 0;
+  // the solid a dose map covers: its engine index, voxel size (zero for a
+  // mesh) and OBB borders (GGEMSSolid::GetVoxelSizes / GetOBBGeometry);
+  // false for navigators that take no dose map
+  virtual bool dose_geometry(int&, float[3], float[3], float[3]) const { return false; }
 };
 
 // GGEMSVoxelizedPhantom + GGEMSVoxelizedSolid (src/navigators/GGEMSVoxelizedPhantom.cc,
@@ -80,6 +84,7 @@ struct GGEMSVoxelizedPhantom : GGEMSNavigator {
   void initialize() override;
   int register_solids(ggh_context* ctx, int material_offset) override;
   void save_results(ggh_context* ctx) override;
+  bool dose_geometry(int& solid, float vs[3], float lo[3], float hi[3]) const override;
 };
 
 // GGEMSMeshedPhantom + GGEMSMeshedSolid (src/navigators/GGEMSMeshedPhantom.cc,
@@ -94,7 +99,8 @@ struct GGEMSMeshedPhantom : GGEMSNavigator {
   int solid_index = -1;
   void initialize() override;
   int register_solids(ggh_context* ctx, int material_offset) override;
-  void save_results(ggh_context*) override {}
+  void save_results(ggh_context* ctx) override;
+  bool dose_geometry(int& solid, float vs[3], float lo[3], float hi[3]) const override;
 };
 
 // GGEMSCTSystem + GGEMSSystem + GGEMSSolidBox (src/navigators/GGEMSCTSystem.cc,
@@ -118,7 +124,7 @@ struct GGEMSCTSystem : GGEMSNavigator {
 // GGEMSDosimetryCalculator (src/navigators/GGEMSDosimetryCalculator.cc)
 struct GGEMSDosimetryCalculator {
   std::string output = "dosi";
-  GGEMSVoxelizedPhantom* navigator = nullptr;
+  GGEMSNavigator* navigator = nullptr;
   float dosel_size[3] = {-1.0f, -1.0f, -1.0f};
   bool photon_tracking = false, edep = false, hit = false, edep_squared = false, uncertainty = false;
   float scale = 1.0f;

@@ -483,9 +483,10 @@ class GGEMS:
         return Engine.attach(_call("get_engine_context_ggems", VP, [VP], self.obj))
 
     def engine_stats(self):
-        out = (C.c_uint64 * 6)()
+        out = (C.c_uint64 * 8)()
         _call("get_engine_stats_ggems", None, [VP, VP], self.obj, C.cast(out, VP))
-        keys = ("primaries", "voxel_steps", "interactions", "detector_steps", "detector_interactions", "hops")
+        keys = ("primaries", "voxel_steps", "interactions", "detector_steps", "detector_interactions", "hops",
+                "mesh_steps", "mesh_interactions")
         return {k: int(v) for k, v in zip(keys, out)}
 
 

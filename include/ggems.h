@@ -231,12 +231,12 @@ char const* get_last_error_ggems(void);
 /* deletes every registered navigator and source and restores the process-table defaults */
 void clean_safely_ggems(void);
 /* wall time of the transport of the last run_ggems (ms) and its engine counters:
- * out[6] = primaries, voxel steps, voxel interactions, detector steps,
- * detector interactions, navigator hops */
+ * out[8] = primaries, voxel steps, voxel interactions, detector steps,
+ * detector interactions, navigator hops, mesh steps, mesh interactions */
 double get_run_time_ms_ggems(GGEMS* ggems);
 /* the engine context (ggh_context*, include/ggems_engine.h) of an initialised GGEMS object */
 void* get_engine_context_ggems(GGEMS* ggems);
-void get_engine_stats_ggems(GGEMS* ggems, uint64_t* out6);
+void get_engine_stats_ggems(GGEMS* ggems, uint64_t* out8);
 /* raw cross-section table row of a material: nbins values, mm^-1 */
 void get_cross_section_table_ggems_cross_sections(GGEMSCrossSections* cross_sections, char const* process_name,
                                                   char const* material_name, float* out, size_t nbins);

@@ -103,7 +103,7 @@ typedef struct {
 } ggh_mesh_solid_desc;
 
 typedef struct {
-  int32_t solid;                 /* voxel solid index returned by ggh_add_voxel_solid */
+  int32_t solid;                 /* voxel or meshed solid (ggh_add_voxel_solid, ggh_add_mesh_solid) */
   float dosel_size[3];
   int32_t n_dosels[3];
   float border_min[3];

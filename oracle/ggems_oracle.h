@@ -100,7 +100,7 @@ typedef struct {
   float voxel_size[3];
   int nvox[3];
   const uint8_t* labels;
-  /* dosimetry (voxel solid only) */
+  /* dosimetry (voxel or meshed solid) */
   int dosimetry;
   float dosel_size[3], inv_dosel[3], dose_bmin[3], dose_bmax[3];
   int ndosel[3];

@@ -625,16 +625,25 @@ static void track_mesh(const ggo_physics* ph, const ggo_solid* s, photon* g, uin
     if (dtri <= next) {
       next = dtri + TOL;
       proc = TRANSPORTATION;
+      if (s->dosimetry && s->photon_tracking) {
+        int id = dosel_index(s, lp);
+        if (id >= 0) __atomic_fetch_add(&s->photon_tracking[id], 1, __ATOMIC_RELAXED);
+      }
     }
     lp = add3(lp, scale3(ld, next));
     if (proc != TRANSPORTATION) {
       if (st) st->mesh_interactions++;
+      float e0 = g->e;
       g->d = ld;
       discrete(ph, g, proc, gmat, rng);
       if (proc == COMPTON || proc == RAYLEIGH) g->scatter = 1;
+      if (s->dosimetry) dose_record(s, e0 - g->e, lp);
       ld = g->d;
     }
-    if (g->e <= ph->photon_cut[gmat]) g->status = DEAD;
+    if (g->e <= ph->photon_cut[gmat]) {
+      if (s->dosimetry) dose_record(s, g->e, lp);
+      g->status = DEAD;
+    }
   } while (g->status == ALIVE);
   float dist = dist_obb(&s->obb, lp, ld);
   g->p = add3(lp, scale3(ld, dist + TOL));
@@ -870,7 +879,9 @@ float ggo_sample_rayleigh(const ggo_physics* ph, int gmat, float e, float* dir, 
   return g.e;
 }
 
-/* compute_dose_ggems_voxelized_solid (ComputeDoseGGEMSVoxelizedSolid.cl:53-121) */
+/* compute_dose_ggems_voxelized_solid (ComputeDoseGGEMSVoxelizedSolid.cl:53-121);
+ * a meshed solid takes its one material (compute_dose_ggems_meshed_solid,
+ * ComputeDoseGGEMSMeshSolid.cl:53-114) */
 void ggo_compute_dose(const ggo_solid* s, const float* density_by_gmat, float scale, int water_ref,
                       float min_density, float* dose, float* unc) {
   int nx = s->ndosel[0], ny = s->ndosel[1], nz = s->ndosel[2];
@@ -886,10 +897,13 @@ void ggo_compute_dose(const ggo_solid* s, const float* density_by_gmat, float sc
     float px = (float)x * s->dosel_size[0] + (float)(nx - 1) * (-0.5f * s->dosel_size[0]);
     float py = (float)y * s->dosel_size[1] + (float)(ny - 1) * (-0.5f * s->dosel_size[1]);
     float pz = (float)z * s->dosel_size[2] + (float)(nz - 1) * (-0.5f * s->dosel_size[2]);
-    int vx = (int)((px - s->obb.bmin[0]) / s->voxel_size[0]);
-    int vy = (int)((py - s->obb.bmin[1]) / s->voxel_size[1]);
-    int vz = (int)((pz - s->obb.bmin[2]) / s->voxel_size[2]);
-    int mat = s->labels[vx + vy * s->nvox[0] + vz * s->nvox[0] * s->nvox[1]];
+    int mat = 0;
+    if (s->type == GGO_SOLID_VOXEL) {
+      int vx = (int)((px - s->obb.bmin[0]) / s->voxel_size[0]);
+      int vy = (int)((py - s->obb.bmin[1]) / s->voxel_size[1]);
+      int vz = (int)((pz - s->obb.bmin[2]) / s->voxel_size[2]);
+      mat = s->labels[vx + vy * s->nvox[0] + vz * s->nvox[0] * s->nvox[1]];
+    }
     float density = water_ref ? 1.0f * U_G_CM3 : density_by_gmat[s->mat_offset + mat];
     dose[gid] = density < min_density ? 0.0f : (float)((double)scale * s->edep[gid] / (double)density / (double)vol / (double)GY);
     if (unc) {

@@ -142,10 +142,12 @@ class Scene:
         self.navigators.append(("ct", [material], dict(modules=mods, threshold=F(threshold),
                                                         store_scatter=store_scatter, n_modules=n_modules)))
 
-    def add_mesh(self, triangles, material, position=(0, 0, 0), rotation=(0, 0, 0), depth=4):
+    def add_mesh(self, triangles, material, position=(0, 0, 0), rotation=(0, 0, 0), depth=4, dosimetry=None):
         """GGEMSMeshedPhantom: triangles [n][3][3] (STL frame) moved by the
         navigator's rotation then position, as GGEMSMeshedSolid::UpdateTriangles
-        does (src/geometries/GGEMSMeshedSolid.cc:205-262), and the GGEMS octree."""
+        does (src/geometries/GGEMSMeshedSolid.cc:205-262), and the GGEMS octree.
+        dosimetry: dict(dosel_size=(x, y, z) mm, photon_tracking) or None; the
+        dose map covers the octree box (GGEMSDosimetryCalculator.cc:445-500)."""
         tr = Transform(AXIS_IDENTITY).rotate(rotation).translate(position)
         m = tr.m
         tri = np.asarray(triangles, F).reshape(-1, 3, 3)
@@ -156,7 +158,8 @@ class Scene:
             acc = (acc + F(m[a, 2]) * tri[:, :, 2]).astype(F)
             out[:, :, a] = (acc + F(m[a, 3])).astype(F)
         mesh = O.mesh_build(out, depth)
-        self.navigators.append(("mesh", [material], dict(tri=out, mesh=mesh)))
+        self.navigators.append(("mesh", [material], dict(tri=out, mesh=mesh, dosimetry=dosimetry,
+                                                          bmin=mesh["obb_min"], bmax=mesh["obb_max"])))
 
     def set_source(self, position, aperture_deg, energies, intensities=None, focal=(0, 0, 0), rotation=(0, 0, 0),
                    mono=False):
@@ -224,22 +227,7 @@ class Scene:
                 nz, ny, nx = pay["labels"].shape
                 s.nvox[:] = [nx, ny, nz]
                 s.labels = O.ptr(pay["labels"])
-                if pay["dosimetry"] is not None:
-                    d = self.dose_grid(pay)
-                    s.dosimetry = 1
-                    s.dosel_size[:] = d["size"].tolist()
-                    s.inv_dosel[:] = (F(1.0) / d["size"]).tolist()
-                    s.dose_bmin[:] = pay["bmin"].tolist()
-                    s.dose_bmax[:] = pay["bmax"].tolist()
-                    s.ndosel[:] = d["n"]
-                    shape = (d["n"][2], d["n"][1], d["n"][0])
-                    tallies["edep"] = np.zeros(shape, np.float64)
-                    tallies["edep2"] = np.zeros(shape, np.float64)
-                    tallies["hit"] = np.zeros(shape, np.int32)
-                    s.edep, s.edep2, s.hit = O.ptr(tallies["edep"]), O.ptr(tallies["edep2"]), O.ptr(tallies["hit"])
-                    if pay["dosimetry"].get("photon_tracking"):
-                        tallies["photon_tracking"] = np.zeros(shape, np.int32)
-                        s.photon_tracking = O.ptr(tallies["photon_tracking"])
+                self._attach_dose(s, pay, tallies)
                 solids.append(s)
             elif kind == "mesh":
                 s = O.Solid()
@@ -249,6 +237,7 @@ class Scene:
                 s.obb.bmax[:] = pay["mesh"]["obb_max"].tolist()
                 s.mat_offset = off
                 s.mesh = O.mesh_struct(pay["mesh"])
+                self._attach_dose(s, pay, tallies)
                 solids.append(s)
             else:
                 for k, mod in enumerate(pay["modules"]):
@@ -279,10 +268,32 @@ class Scene:
         src.is_interp = S["is_interp"]
         return ph, (O.Solid * len(solids))(*solids), src
 
+    def _attach_dose(self, s, pay, tallies):
+        if pay["dosimetry"] is None:
+            return
+        d = self.dose_grid(pay)
+        s.dosimetry = 1
+        s.dosel_size[:] = d["size"].tolist()
+        s.inv_dosel[:] = (F(1.0) / d["size"]).tolist()
+        s.dose_bmin[:] = pay["bmin"].tolist()
+        s.dose_bmax[:] = pay["bmax"].tolist()
+        s.ndosel[:] = d["n"]
+        shape = (d["n"][2], d["n"][1], d["n"][0])
+        tallies["edep"] = np.zeros(shape, np.float64)
+        tallies["edep2"] = np.zeros(shape, np.float64)
+        tallies["hit"] = np.zeros(shape, np.int32)
+        s.edep, s.edep2, s.hit = O.ptr(tallies["edep"]), O.ptr(tallies["edep2"]), O.ptr(tallies["hit"])
+        if pay["dosimetry"].get("photon_tracking"):
+            tallies["photon_tracking"] = np.zeros(shape, np.int32)
+            s.photon_tracking = O.ptr(tallies["photon_tracking"])
+
     def dose_grid(self, pay):
-        # GGEMSDosimetryCalculator::Initialize with dosel size == voxel size (:445-487)
-        size = pay["vs"]
-        n = [int(F(pay["bmax"][i] - pay["bmin"][i]) / size[i]) for i in range(3)]
+        # GGEMSDosimetryCalculator::Initialize (:445-500): dosel size = voxel
+        # size of a voxelized phantom, or the one set (required for a mesh)
+        size = pay.get("vs")
+        if pay["dosimetry"].get("dosel_size") is not None:
+            size = np.array(pay["dosimetry"]["dosel_size"], F)
+        n = [int(F(F(pay["bmax"][i] - pay["bmin"][i]) / size[i])) for i in range(3)]
         return dict(size=size, n=n)
 
     def prepare_oracle(self):
@@ -328,6 +339,10 @@ class Scene:
             elif kind == "mesh":
                 k = eng.add_mesh_solid(pay["tri"], pay["mesh"]["obb_min"], pay["mesh"]["obb_max"], off)
                 self.engine_solids.setdefault("mesh", []).append(k)
+                if pay["dosimetry"] is not None:
+                    d = self.dose_grid(pay)
+                    eng.set_dosimetry(k, d["size"], d["n"], pay["bmin"], pay["bmax"], True, True,
+                                      bool(pay["dosimetry"].get("photon_tracking")))
             else:
                 for mod in pay["modules"]:
                     k = eng.add_box_solid(mod["m"], mod["bmin"], mod["bmax"], mod["ne"], mod["size"],

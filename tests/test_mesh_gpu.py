@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import ggems_oracle as O
-from ggems_amd import authoring
+from ggems_amd import authoring, configs
 from scene import Scene
 
 pytestmark = pytest.mark.gpu
@@ -79,3 +79,60 @@ def test_mesh_transport_matches_oracle(curved):
     h_ref = np.concatenate([h.ravel() for h in ref["hist"]]).astype(np.int64)
     h_got = np.concatenate([h.ravel() for h in got["hist"]]).astype(np.int64)
     assert np.abs(h_ref - h_got).sum() <= 0.02 * h_ref.sum(), (np.abs(h_ref - h_got).sum(), h_ref.sum())
+
+
+def _mesh_ct_oracle(workdir, water_box, dosimetry):
+    """The oracle's copy of configs.mesh_ct: navigators in the same order."""
+    outer, inner, box, _ = configs.mesh_ct_inputs(workdir)
+    sc = Scene()
+    sc.add_mesh(authoring.read_stl(outer), "SoftTissus_15", depth=4,
+                dosimetry={"dosel_size": (1.0, 1.0, 1.0)} if dosimetry else None)
+    sc.add_mesh(authoring.read_stl(inner), "RibBone", depth=4)
+    if water_box:
+        lab, spacing = authoring.read_mhd(box)
+        sc.add_phantom(lab, ["Water"], tuple(spacing))
+    sc.add_ct_curved("GOS", configs.MESH_MODULES, configs.MESH_ELEMENTS, (1.0, 1.0, 1.0), configs.MESH_SID,
+                     configs.MESH_SDD, 10e-3)
+    sc.set_source((-configs.MESH_SID, 0.0, 0.0), 5.0, [0.08], mono=True)
+    return sc.build()
+
+
+@pytest.mark.parametrize("water_box", [False, True])
+def test_mesh_ct_script_matches_oracle(tmp_path, water_box):
+    """configs.mesh_ct through the GGEMS API: the curved projection (888
+    columns x 64 rows), the run statistics and, without the box, the dose
+    map on the outer mesh.  With the enclosing water box no photon reaches a
+    mesh, in both implementations."""
+    n = 300_000
+    wd = str(tmp_path)
+    keep = configs.mesh_ct(wd, n_particles=n, water_box=water_box, dosimetry=not water_box)
+    g = configs.run(seed=4242)
+    stats = g.engine_stats()
+    proj, spacing = authoring.read_mhd(os.path.join(wd, "projection.mhd"))
+    assert proj.shape == (1, 888, 64)
+    ref = _mesh_ct_oracle(wd, water_box, not water_box).run_oracle(n, 4242)
+    rs = ref["stats"]
+    assert stats["primaries"] == rs["primaries"] == n
+    if water_box:
+        assert stats["mesh_steps"] == rs["mesh_steps"] == 0
+        assert abs(stats["voxel_steps"] - rs["voxel_steps"]) <= 0.01 * rs["voxel_steps"]
+    else:
+        for k in ("mesh_steps", "mesh_interactions", "hops"):
+            assert abs(stats[k] - rs[k]) <= 0.01 * rs[k] + 10, (k, stats[k], rs[k])
+    h_ref = np.concatenate(ref["hist"], axis=0)
+    assert h_ref.shape == (888, 64)
+    l1 = np.abs(proj[0].astype(np.int64) - h_ref).sum()
+    assert l1 <= 0.03 * h_ref.sum() + 20, (l1, h_ref.sum())
+    if not water_box:
+        edep, _ = authoring.read_mhd(os.path.join(wd, "dosimetry_edep.mhd"))
+        hit, _ = authoring.read_mhd(os.path.join(wd, "dosimetry_hit.mhd"))
+        dose, _ = authoring.read_mhd(os.path.join(wd, "dosimetry_dose.mhd"))
+        assert edep.shape == ref["edep"].shape == (106, 106, 106)  # octree box +-53 mm, 1 mm dosels
+        assert abs(edep.sum() - ref["edep"].sum()) <= 0.01 * ref["edep"].sum()
+        assert np.abs(hit.astype(np.int64) - ref["hit"]).sum() <= 0.03 * ref["hit"].sum()
+        # one material: dose is edep over the soft-tissue density everywhere
+        nz = edep > 0
+        ratio = dose[nz] / edep[nz]
+        assert np.allclose(ratio, ratio[0], rtol=1e-5)
+    g.delete()
+    del keep

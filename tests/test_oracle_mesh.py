@@ -3,6 +3,7 @@ ray/mesh intersection list and the mesh transport, against geometry that
 has closed forms (spheres, exact rational ray/triangle intersection,
 Beer-Lambert along the mesh's own chord)."""
 
+import ctypes as C
 import math
 from fractions import Fraction
 
@@ -138,3 +139,60 @@ def test_beer_lambert_through_mesh_chord():
     sd = math.sqrt(n * p * (1 - p))
     assert abs(counts - n * p) < 4 * sd, (counts, n * p, sd)
     assert r["stats"]["mesh_steps"] > 0
+
+
+def test_mesh_dose_photoelectric_only():
+    """Dose map on a meshed solid, photoelectric absorption only, pencil beam
+    through the centre of a water sphere: every absorbed photon leaves its
+    whole energy in one dosel on the beam axis and is recorded twice (the
+    deposit, then the zero-energy record of the energy cut), so
+    edep = E0 * absorbed and hits = 2 * absorbed exactly; absorbed photons
+    follow Beer-Lambert along the chord, their depths the truncated
+    exponential; the dose is edep over the one material's density."""
+    R, e0, n = 20.0, 0.040, 100_000
+    sc = Scene(processes=(1,))
+    sc.add_mesh(authoring.icosphere(R, 4), "Water", rotation=(0.31, 0.17, 0.11), depth=4,
+                dosimetry={"dosel_size": (1.0, 1.0, 1.0)})
+    sc.add_ct_flat("GOS", (1, 1), (8, 8, 1), (2.0, 2.0, 30.0), 600.0, 1000.0, 1e-3, store_scatter=False)
+    sc.set_source((-600.0, 0.0, 0.0), 0.005, [e0], mono=True)
+    sc.build()
+    t = {}
+    ph, solids, src = sc._oracle_structs(t)
+    st = O.Stats()
+    O.lib().ggo_run(C.byref(ph), solids, len(solids), C.byref(src), n, 99, 4, C.byref(st))
+    transmitted = int(t["hist"][0].sum())
+    absorbed = n - transmitted
+    edep, hit = t["edep"], t["hit"]
+    assert edep.shape == (42, 42, 42)  # octree box +-21 mm
+    assert edep.sum() == absorbed * float(F(e0))
+    assert hit.sum() == 2 * absorbed
+    # deposits on the beam axis only: dosels y, z = 21 (the axis is the face
+    # between dosels 20 and 21 grazed by the 0.005 degree cone: 20 or 21)
+    iz, iy, ix = np.nonzero(hit)
+    assert set(iy) <= {20, 21} and set(iz) <= {20, 21}
+    m = sc.navigators[0][2]["mesh"]
+    k, pts = O.mesh_hits(O.mesh_struct(m), np.array([-100.0, 0, 0], F), np.array([1.0, 0, 0], F))
+    chord = float(np.linalg.norm(pts[1].astype(np.float64) - pts[0]))
+    eid = int(np.searchsorted(sc.ebins, F(e0), side="right") - 1)
+    mu = float(sc.cs[1, 0, eid])
+    p = 1.0 - math.exp(-mu * chord)
+    sd = math.sqrt(n * p * (1 - p))
+    assert abs(absorbed - n * p) < 4 * sd, (absorbed, n * p, sd)
+    # mean depth of absorption past the entry surface, by dosel centre
+    x_entry = float(pts[0][0])
+    depth = (np.arange(42) + 0.5 - 21.0) - x_entry
+    per_x = edep.sum(axis=(0, 1)) / float(F(e0))
+    mean = float((per_x * depth).sum() / per_x.sum())
+    L = chord
+    expect = 1.0 / mu - L * math.exp(-mu * L) / (1.0 - math.exp(-mu * L))
+    assert abs(mean - expect) < 0.5 + 4 * L / math.sqrt(12 * absorbed), (mean, expect)
+    # dose: edep over the water density and the dosel volume
+    dens = sc.arrays["density"].astype(F)
+    dose = np.zeros(edep.size, F)
+    unc = np.zeros(edep.size, F)
+    O.lib().ggo_compute_dose(C.byref(solids[0]), O.ptr(dens), F(1.0), 0, F(0.0), O.ptr(dose), O.ptr(unc))
+    mev_j, kg = 1.602176634e-13, 1e-6  # 1 mm^3 of water (1 g/cm3)
+    e = edep.reshape(-1)
+    nz = e > 0
+    assert np.allclose(dose[nz] / (e[nz] * mev_j / kg), 1.0, rtol=1e-5)
+    assert (dose[~nz] == 0).all()
