@@ -865,7 +865,10 @@ __device__ inline void cold_step(const RunParams& P, const Tabs& T, Lane& L, Vox
       L.ray.set(g2l_dir(s.m, L.ray.d));
       if (!kSingle) G = s.vox;
       L.mode = kInVoxel;
-    } else if (s.type == kMesh) {
+    } else if (!kSingle && s.type == kMesh) {
+      // ggh_run launches the single-voxel-solid variant only on scenes
+      // without meshes, which keeps the mesh loop (and even this test) out
+      // of its register allocation
       track_mesh(P, T, s, L);
     } else {
       track_box(P, T, s, L, c);
@@ -875,7 +878,7 @@ __device__ inline void cold_step(const RunParams& P, const Tabs& T, Lane& L, Vox
 
 // NP: activated processes (-1: read at run time); kSingle: the scene has
 // exactly one voxel solid, whose geometry the voxel step reads from the
-// kernel arguments; kLDS: step tables staged in LDS; kTrack: photon
+// kernel arguments, and no meshed solid; kLDS: step tables staged in LDS; kTrack: photon
 // fluence tracking may be on.
 template <int NP, bool kSingle, bool kLDS, bool kTrack>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWavesPerEU)))
@@ -1243,9 +1246,9 @@ VoxGeom make_vox(const ggh_voxel_solid_desc* d, int solid) {
   return G;
 }
 
-int n_voxel_solids(const ggh_context* c) {
+int n_solids_of(const ggh_context* c, int32_t type) {
   int n = 0;
-  for (const DevSolid& s : c->solids) n += s.type == kVoxel;
+  for (const DevSolid& s : c->solids) n += s.type == type;
   return n;
 }
 
@@ -1773,12 +1776,13 @@ int ggh_run(ggh_context* c, uint64_t n) {
   P.base = (uint32_t)base;
   P.rem = (uint32_t)rem;
   P.n_slots = (uint32_t)slots;
-  const bool single = n_voxel_solids(c) == 1;
+  const bool single = n_solids_of(c, kVoxel) == 1 && n_solids_of(c, kMesh) == 0;
   const size_t lds = tables_lds_bytes(c);
   const bool use_lds = lds <= (size_t)kLdsBudget;
   const size_t shmem = use_lds ? lds : 0;
   // kernel variants: the specialised one (three activated processes, one
-  // voxel solid, tables in LDS, no fluence tracking) and the general one
+  // voxel solid and no mesh, tables in LDS, no fluence tracking) and the
+  // general one
   using Kernel = void (*)(RunParams);
   const bool special = c->n_proc == 3 && single && use_lds && !c->ptrack.p && !c->force_general;
   const Kernel fn = special ? transport_kernel<3, true, true, false>
