@@ -7,6 +7,11 @@ voxel_ct   the 512x512x256 body of 0.8 mm voxels (air, lung, adipose,
            soft tissue, cortical bone; 40 seeded ellipsoids) with the cfg1
            source/detector geometry, a 120 kVp Kramers spectrum with 2.5 mm Al
            filtration and FP64 dose + uncertainty tallies.
+acquisition  a CT acquisition in one launch: n_angles gantry positions (1
+           degree apart by default) around a seeded ellipsoid phantom of
+           side^3 1 mm voxels, per-projection photon budgets drawn uniformly
+           in a range, a flat 512x384 detector of 0.78 mm pixels at SDD
+           1000 mm (SID 600 mm) and the 120 kVp spectrum.
 mesh_ct    two meshed phantoms (an icosphere of radius 50 mm and 81,920
            triangles in soft tissue, one of radius 15 mm in cortical bone) in
            a 300 mm water box, an 80 keV point cone beam at SID 600 mm and a
@@ -234,6 +239,73 @@ def mesh_ct(workdir, n_particles=100_000_000, water_box=True, dosimetry=False, o
     src.set_monoenergy(80.0, "keV")
     keep.append(src)
     return keep
+
+
+ACQ_DETECTOR = (384, 512)  # rows (axial, local x) x columns (transaxial, local y)
+ACQ_PIXEL = 0.78
+
+
+def acquisition_inputs(workdir, side=128, seed=1):
+    """The ellipsoid phantom of side^3 1 mm voxels (body and ellipsoids scaled
+    to the side), its range file and the 120 kVp spectrum (cached)."""
+    os.makedirs(workdir, exist_ok=True)
+    mhd = os.path.join(workdir, "ellipsoids_%d_%d.mhd" % (seed, side))
+    rng = os.path.join(workdir, "range_body.txt")
+    spec = os.path.join(workdir, "spectrum_120kVp_2.5mmAl.dat")
+    if not os.path.exists(mhd):
+        lab = authoring.ellipsoid_body_phantom(seed, shape=(side, side, side), voxel=1.0, n_ellipsoids=40,
+                                               body_axes=(0.45 * side, 0.35 * side),
+                                               semi_axis_range=(side / 50.0, side / 4.0))
+        authoring.write_mhd(mhd, lab, (1.0, 1.0, 1.0))
+    authoring.write_range(rng, authoring.BODY_MATERIALS)
+    if not os.path.exists(spec):
+        GGEMSMaterialsDatabaseManager().set_materials(MATERIALS_TXT)
+        e, w = authoring.kramers_spectrum(120.0, 20.0, 2.5, authoring.aluminium_mu())
+        authoring.write_spectrum(spec, e, w)
+    return mhd, rng, spec
+
+
+def acquisition_plan(n_angles=360, budget_range=(1_000_000, 30_000_000), seed=5, step_deg=1.0):
+    """Gantry rotations about z (degrees) and seeded per-projection budgets."""
+    import numpy as np
+    r = np.zeros((n_angles, 3), np.float32)
+    r[:, 2] = np.arange(n_angles, dtype=np.float32) * np.float32(step_deg)
+    budgets = np.random.default_rng(seed).integers(budget_range[0], budget_range[1] + 1, n_angles).astype(np.uint64)
+    return r, budgets
+
+
+def acquisition(workdir, side=128, seed=1):
+    """The acquisition scene (phantom, flat panel, source); transport with
+    GGEMS().run_acquisition(rotations, budgets) after initialize."""
+    mhd, rng, spec = acquisition_inputs(workdir, side, seed)
+    setup(workdir)
+    phantom = GGEMSVoxelizedPhantom("phantom")
+    phantom.set_phantom(mhd, rng)
+    phantom.set_rotation(0.0, 0.0, 0.0, "deg")
+    phantom.set_position(0.0, 0.0, 0.0, "mm")
+    ct = GGEMSCTSystem("detector")
+    ct.set_ct_type("flat")
+    ct.set_number_of_modules(1, 1)
+    ct.set_number_of_detection_elements(ACQ_DETECTOR[0], ACQ_DETECTOR[1], 1)
+    ct.set_size_of_detection_elements(ACQ_PIXEL, ACQ_PIXEL, 1.0, "mm")
+    ct.set_material("GOS")
+    ct.set_source_detector_distance(SDD, "mm")
+    ct.set_source_isocenter_distance(SID, "mm")
+    ct.set_rotation(0.0, 0.0, 0.0, "deg")
+    ct.set_threshold(10.0, "keV")
+    ct.save(os.path.join(workdir, "projections"))
+    ct.store_scatter(True)
+    _common_physics()
+    src = GGEMSXRaySource("point_source")
+    src.set_source_particle_type("gamma")
+    src.set_number_of_particles(1)
+    src.set_position(-SID, 0.0, 0.0, "mm")
+    src.set_rotation(0.0, 0.0, 0.0, "deg")
+    half_diag = 0.5 * ACQ_PIXEL * math.hypot(*ACQ_DETECTOR)
+    src.set_beam_aperture(math.degrees(math.atan(half_diag / SDD)), "deg")
+    src.set_focal_spot_size(0.0, 0.0, 0.0, "mm")
+    src.set_polyenergy(spec)
+    return [phantom, ct, src]
 
 
 def run(seed=777):

@@ -83,7 +83,8 @@ struct DevSolid {
   const float4* bvh;
   const float4* tris;
   int32_t n_bvh;
-  int32_t pad_[3];
+  int32_t box_index;  // rank among the box solids (per-projection matrices), -1 otherwise
+  int32_t pad_[2];
 };
 
 static_assert(sizeof(DevSolid) % 16 == 0, "solids are staged in LDS after 16-byte table entries");
@@ -134,6 +135,15 @@ struct RunParams {
   int32_t resume_threshold;
   int32_t interact_batch;
   VoxGeom vox0;               // the only voxel solid, when there is exactly one
+  // CT acquisition: projection p places the source at acq_src[p] and box
+  // solid j at acq_box[p * n_box + j]; its primaries are laid out in
+  // batches as one run of that many primaries ({n_batches, base, rem}); the
+  // histograms of box solids hold n_proj consecutive images
+  const Mat34* acq_src;
+  const Mat34* acq_box;
+  const uint4* acq_layout;
+  int32_t n_proj;
+  int32_t n_box;
 };
 
 // tables as seen by a block: LDS copies or the global arrays
@@ -175,6 +185,7 @@ struct Lane {
   int32_t pend;
   uint32_t todo;
   uint32_t slot;
+  uint32_t proj;  // current projection of an acquisition
 };
 
 struct Counters {
@@ -480,10 +491,14 @@ __device__ inline bool walk_step(const VoxGeom& G, f3& p, const Ray& ray, int32_
 // ---------------------------------------------------------------------------
 // CT module transport (src/kernels/TrackThroughGGEMSSolidBox.cl:56-261)
 // ---------------------------------------------------------------------------
+// kAcq: the module sits where the lane's projection puts it and scores into
+// that projection's image.
+template <bool kAcq>
 __device__ void track_box(const RunParams& P, const Tabs& T, const DevSolid& s, Lane& L, Counters& c) {
-  f3 lp = g2l_pos(s.m, L.p);
+  const Mat34& m = kAcq ? P.acq_box[L.proj * P.n_box + s.box_index] : s.m;
+  f3 lp = g2l_pos(m, L.p);
   Ray ld;
-  ld.set(g2l_dir(s.m, L.ray.d));
+  ld.set(g2l_dir(m, L.ray.d));
   const Box b = s.border;
   const int gmat = s.mat_offset;
   bool alive = true;
@@ -519,15 +534,16 @@ __device__ void track_box(const RunParams& P, const Tabs& T, const DevSolid& s, 
         int ix = (int)((lp.x - b.lo.x) / esx);
         int iy = (int)((lp.y - b.lo.y) / esy);
         if (ix >= 0 && ix < s.ne[0] && iy >= 0 && iy < s.ne[1]) {
-          int id = ix + iy * s.ne[0];
+          // image of the lane's projection (acquisitions stack them)
+          const size_t id = (size_t)(ix + iy * s.ne[0]) + (kAcq ? (size_t)L.proj * s.ne[0] * s.ne[1] : 0);
           atomicAdd(&s.histogram[id], 1);
           if (s.scatter && L.scatter) atomicAdd(&s.scatter[id], 1);
         }
       }
     }
   } while (alive);
-  L.p = l2g_pos(s.m, lp);
-  L.ray.d = l2g_dir(s.m, ld.d);
+  L.p = l2g_pos(m, lp);
+  L.ray.d = l2g_dir(m, ld.d);
   L.mode = alive ? kHop : kNeedPrimary;
 }
 
@@ -687,8 +703,8 @@ __device__ void track_mesh(const RunParams& P, const Tabs& T, const DevSolid& s,
 // ---------------------------------------------------------------------------
 // source (src/kernels/GetPrimariesGGEMSXRaySource.cl:52-143)
 // ---------------------------------------------------------------------------
-__device__ void sample_primary(const RunParams& P, const Tabs& T, Rng& r, f3& pos, f3& dir, float& energy,
-                               int& eid) {
+__device__ void sample_primary(const RunParams& P, const Tabs& T, const Mat34& src_m, Rng& r, f3& pos, f3& dir,
+                               float& energy, int& eid) {
   double phi = r.uniform();
   double theta = r.uniform();
   phi *= (double)kTwoPi;
@@ -696,13 +712,13 @@ __device__ void sample_primary(const RunParams& P, const Tabs& T, Rng& r, f3& po
   theta = acos(1.0 - new_aperture * theta);
   double st = sin(theta);
   f3 rotation = mk((float)(cos(phi) * st), (float)(sin(phi) * st), (float)cos(theta));
-  f3 gp = l2g_pos(P.src_m, mk(0.0f, 0.0f, 0.0f));
+  f3 gp = l2g_pos(src_m, mk(0.0f, 0.0f, 0.0f));
   f3 d = normalize(mk(0.0f - gp.x, 0.0f - gp.y, 0.0f - gp.z));
   d = normalize(rotate_unit_z(rotation, d));
   gp.x = P.focal.x * (r.uniform() - 0.5f);
   gp.y = P.focal.y * (r.uniform() - 0.5f);
   gp.z = P.focal.z * (r.uniform() - 0.5f);
-  gp = l2g_pos(P.src_m, gp);
+  gp = l2g_pos(src_m, gp);
   int idx = 0;
   float rnd = 0.0f;
   if (P.n_energy_arg != 1) {
@@ -723,19 +739,26 @@ __device__ void sample_primary(const RunParams& P, const Tabs& T, Rng& r, f3& po
 // (src/kernels/ParticleSolidDistanceGGEMSVoxelizedSolid.cl:43-108) followed by
 // project_to_* (src/kernels/ProjectToGGEMSVoxelizedSolid.cl:44-129).  Returns
 // the solid the photon now sits in, or -1 when it leaves the world.
-__device__ int find_and_project(const RunParams& P, const Tabs& T, f3& pos, f3 dir) {
+template <bool kAcq>
+__device__ inline const Mat34& solid_matrix(const RunParams& P, const DevSolid& s, uint32_t proj) {
+  return (kAcq && s.type == kBoxSolid) ? P.acq_box[proj * P.n_box + s.box_index] : s.m;
+}
+
+template <bool kAcq>
+__device__ int find_and_project(const RunParams& P, const Tabs& T, f3& pos, f3 dir, uint32_t proj) {
   float dist = kOutOfWorld;
   int solid = -1;
   for (int k = 0; k < P.n_solids; ++k) {
     const DevSolid& s = T.solids[k];
-    f3 lp = g2l_pos(s.m, pos);
+    const Mat34& m = solid_matrix<kAcq>(P, s, proj);
+    f3 lp = g2l_pos(m, pos);
     if (in_aabb(lp, s.border)) {
       dist = 0.0f;
       solid = k;
       break;
     }
     Ray lr;
-    lr.set(g2l_dir(s.m, dir));
+    lr.set(g2l_dir(m, dir));
     float d = distance_to_aabb(lp, lr, s.border);
     if (d < dist) {
       dist = d;
@@ -744,10 +767,11 @@ __device__ int find_and_project(const RunParams& P, const Tabs& T, f3& pos, f3 d
   }
   if (solid == -1 || dist == kOutOfWorld) return -1;
   const DevSolid& s = T.solids[solid];
+  const Mat34& m = solid_matrix<kAcq>(P, s, proj);
   pos = add(pos, scl(dir, dist + kTol));
-  f3 lp = g2l_pos(s.m, pos);
+  f3 lp = g2l_pos(m, pos);
   safety_inside(lp, s.border);
-  pos = l2g_pos(s.m, lp);
+  pos = l2g_pos(m, lp);
   return solid;
 }
 
@@ -794,9 +818,25 @@ __device__ inline void resolve_interaction(const RunParams& P, const Tabs& T, La
   L.mode = alive ? kInVoxel : kNeedPrimary;
 }
 
+// primaries of particle slot s in a run laid out as {n_batches, base, rem}:
+// batch b holds base + (b < rem) photons and slot s takes photon s of each
+__device__ inline uint32_t slot_todo(uint32_t n_batches, uint32_t base, uint32_t rem, uint32_t s) {
+  return s < base ? n_batches : (s == base ? rem : 0u);
+}
+
+// an acquisition slot runs its primaries of projection 0, then of 1, ...:
+// its random stream continues exactly as over consecutive runs
+__device__ inline void next_projection(const RunParams& P, Lane& L) {
+  while (L.todo == 0 && L.proj + 1 < (uint32_t)P.n_proj) {
+    ++L.proj;
+    const uint4 lay = P.acq_layout[L.proj];
+    L.todo = slot_todo(lay.x, lay.y, lay.z, L.slot);
+  }
+}
+
 // the cold work of one lane, one state at a time in an order that lets a
 // lane pass through several states in one call
-template <bool kSingle>
+template <bool kSingle, bool kAcq>
 __device__ inline void cold_step(const RunParams& P, const Tabs& T, Lane& L, VoxGeom& G, Counters& c) {
   if (L.mode == kInteract) resolve_interaction(P, T, L, c);
   if (L.mode == kCut) {
@@ -809,6 +849,7 @@ __device__ inline void cold_step(const RunParams& P, const Tabs& T, Lane& L, Vox
     L.ray.d = l2g_dir(s.m, L.ray.d);
     L.mode = kHop;
   }
+  if (kAcq && L.mode == kNeedPrimary) next_projection(P, L);
   if (L.mode == kNeedPrimary && L.todo == 0) {
     if (L.slot != 0xffffffffu) store_stream(P, L.slot, L.r);
     L.mode = kNeedSlot;
@@ -826,8 +867,14 @@ __device__ inline void cold_step(const RunParams& P, const Tabs& T, Lane& L, Vox
     if (my < P.n_slots) {
       L.slot = my;
       load_stream(P, my, L.r);
-      // batch b holds base + (b < rem) photons; slot s takes photon s of each
-      L.todo = my < P.base ? P.n_batches : (my == P.base ? P.rem : 0u);
+      if (kAcq) {
+        const uint4 lay = P.acq_layout[0];
+        L.proj = 0;
+        L.todo = slot_todo(lay.x, lay.y, lay.z, my);
+        next_projection(P, L);
+      } else {
+        L.todo = slot_todo(P.n_batches, P.base, P.rem, my);
+      }
       L.mode = kNeedPrimary;
     } else {
       L.slot = 0xffffffffu;
@@ -839,7 +886,7 @@ __device__ inline void cold_step(const RunParams& P, const Tabs& T, Lane& L, Vox
       L.mode = kNeedSlot;  // empty slot: take another one on the next pass
     } else {
       --L.todo;
-      sample_primary(P, T, L.r, L.p, L.ray.d, L.e, L.eid);
+      sample_primary(P, T, kAcq ? P.acq_src[L.proj] : P.src_m, L.r, L.p, L.ray.d, L.e, L.eid);
       ++c.primaries;
       L.scatter = 0;
       L.hop = 0;
@@ -853,7 +900,7 @@ __device__ inline void cold_step(const RunParams& P, const Tabs& T, Lane& L, Vox
     }
     ++L.hop;
     ++c.hops;
-    const int k = find_and_project(P, T, L.p, L.ray.d);
+    const int k = find_and_project<kAcq>(P, T, L.p, L.ray.d, L.proj);
     if (k < 0) {
       L.mode = kNeedPrimary;
       return;
@@ -871,7 +918,7 @@ __device__ inline void cold_step(const RunParams& P, const Tabs& T, Lane& L, Vox
       // of its register allocation
       track_mesh(P, T, s, L);
     } else {
-      track_box(P, T, s, L, c);
+      track_box<kAcq>(P, T, s, L, c);
     }
   }
 }
@@ -879,8 +926,9 @@ __device__ inline void cold_step(const RunParams& P, const Tabs& T, Lane& L, Vox
 // NP: activated processes (-1: read at run time); kSingle: the scene has
 // exactly one voxel solid, whose geometry the voxel step reads from the
 // kernel arguments, and no meshed solid; kLDS: step tables staged in LDS; kTrack: photon
-// fluence tracking may be on.
-template <int NP, bool kSingle, bool kLDS, bool kTrack>
+// fluence tracking may be on; kAcq: a CT acquisition (per-projection source
+// and module placement).
+template <int NP, bool kSingle, bool kLDS, bool kTrack, bool kAcq>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWavesPerEU)))
 void transport_kernel(RunParams P) {
   extern __shared__ float4 s_tab[];
@@ -931,6 +979,7 @@ void transport_kernel(RunParams P) {
   L.pend = 0;
   L.todo = 0;
   L.slot = 0xffffffffu;
+  L.proj = 0;
   VoxGeom G = P.vox0;
   Counters c = {0, 0, 0, 0, 0};
   unsigned long long steps = 0;  // wave-uniform
@@ -943,7 +992,7 @@ void transport_kernel(RunParams P) {
     for (;;) {
       const bool cold = L.mode != kInVoxel && L.mode != kDone;
       if (!__any(cold) || __popcll(__ballot(L.mode == kInVoxel)) >= P.resume_threshold) break;
-      if (cold) cold_step<kSingle>(P, T, L, G, c);
+      if (cold) cold_step<kSingle, kAcq>(P, T, L, G, c);
     }
     // ---- hot phase: voxel steps while enough lanes are inside a phantom
     uint64_t act = __ballot(L.mode == kInVoxel);
@@ -1070,6 +1119,11 @@ __global__ void mesh_hits_kernel(RunParams P, int solid, int n_rays, const float
   o[3] = q1.x; o[4] = q1.y; o[5] = q1.z;
 }
 
+// slot-queue reset as a kernel node: captured graphs hold no memset nodes
+__global__ void reset_queue_kernel(uint32_t* queue) {
+  if (threadIdx.x == 0) *queue = 0u;
+}
+
 __global__ void primaries_kernel(RunParams P, int n, float* pos, float* dir, float* e) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
@@ -1079,7 +1133,7 @@ __global__ void primaries_kernel(RunParams P, int n, float* pos, float* dir, flo
   float energy;
   int eid;
   const Tabs T = {P.tab_sig, P.tab_rcp, P.ebins, P.solids, P.spectrum, P.cdf};
-  sample_primary(P, T, r, gp, gd, energy, eid);
+  sample_primary(P, T, P.src_m, r, gp, gd, energy, eid);
   pos[3 * s] = gp.x; pos[3 * s + 1] = gp.y; pos[3 * s + 2] = gp.z;
   dir[3 * s] = gd.x; dir[3 * s + 1] = gd.y; dir[3 * s + 2] = gd.z;
   e[s] = energy;
@@ -1205,6 +1259,15 @@ struct ggh_context {
   bool force_general = false;
   unsigned last_grid = 0;
   int last_variant = -1;
+  // CT acquisition
+  int acq_n_proj = 0;
+  uint32_t acq_slots = 0;
+  DevBuf<Mat34> acq_src, acq_box;
+  DevBuf<uint4> acq_layout;
+  std::vector<DevBuf<int32_t>> acq_hist, acq_scat;  // per solid: n_proj images (box solids)
+  DevBuf<DevSolid> acq_solids;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
 };
 
 namespace {
@@ -1317,12 +1380,130 @@ uint32_t urandom_seed() {
   return s;
 }
 
+void drop_graph(ggh_context* c) {
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+}
+
+void drop_acquisition(ggh_context* c) {
+  drop_graph(c);
+  c->acq_src.release();
+  c->acq_box.release();
+  c->acq_layout.release();
+  c->acq_solids.release();
+  for (auto& b : c->acq_hist) b.release();
+  for (auto& b : c->acq_scat) b.release();
+  c->acq_hist.clear();
+  c->acq_scat.clear();
+  c->acq_n_proj = 0;
+  c->acq_slots = 0;
+}
+
 // bytes of the LDS image the transport kernel stages per block: step tables,
 // solids, energy bins, spectrum and cdf (n_energy_arg + 1 entries each, as
 // ggh_set_xray_source stores them)
 size_t tables_lds_bytes(const ggh_context* c) {
   return (size_t)2 * c->n_mat * c->nbins * sizeof(float4) + c->solids.size() * sizeof(DevSolid) +
          (size_t)c->nbins * sizeof(float) + (size_t)2 * (c->n_energy_arg + 1) * sizeof(float);
+}
+
+// everything a transport launch needs, checked; uploads the solids
+int check_ready(ggh_context* c) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!c->have_physics) return fail(-3, "physics tables not set");
+  if (!c->have_source) return fail(-3, "source not set");
+  if (!c->seeded) return fail(-3, "particle slots not seeded");
+  if (c->solids.empty()) return fail(-3, "no solid registered");
+  for (size_t i = 0; i < c->solids.size(); ++i)
+    if (c->solids[i].mat_offset < 0 || c->max_material[i] >= c->n_mat)
+      return fail(-3, "solid labels reach a material without physics tables");
+  return sync_solids(c);
+}
+
+// GGEMSSource::OrganizeParticlesInBatch (src/sources/GGEMSSource.cc:403-425):
+// ceil(float(n) / slots) batches of n / batches photons, the first n % batches
+// batches one more; slots = the largest batch
+int batch_layout(uint64_t n, uint32_t& n_batches, uint32_t& base, uint32_t& rem, uint32_t& slots) {
+  uint64_t nb = (uint64_t)std::ceil((float)n / (float)GGH_MAX_SLOTS);
+  if (nb == 0) nb = 1;
+  const uint64_t b = nb == 1 ? n : n / nb;
+  const uint64_t r = nb == 1 ? 0 : n % nb;
+  const uint64_t sl = b + (r ? 1 : 0);
+  if (sl > GGH_MAX_SLOTS) return fail(-3, "batch larger than the slot buffer");
+  if (nb > 0xffffffffull) return fail(-3, "too many batches");
+  n_batches = (uint32_t)nb;
+  base = (uint32_t)b;
+  rem = (uint32_t)r;
+  slots = (uint32_t)sl;
+  return 0;
+}
+
+struct Launch {
+  void (*fn)(RunParams);
+  unsigned grid;
+  size_t shmem;
+};
+
+// kernel variant and persistent grid: the specialised variant (three
+// activated processes, one voxel solid and no mesh, tables in LDS, no fluence
+// tracking) or the general one, each with or without acquisition placement
+int pick_launch(ggh_context* c, uint64_t slots, bool acq, Launch& L) {
+  const bool single = n_solids_of(c, kVoxel) == 1 && n_solids_of(c, kMesh) == 0;
+  const size_t lds = tables_lds_bytes(c);
+  const bool use_lds = lds <= (size_t)kLdsBudget;
+  L.shmem = use_lds ? lds : 0;
+  const bool special = c->n_proc == 3 && single && use_lds && !c->ptrack.p && !c->force_general;
+  if (acq)
+    L.fn = special ? transport_kernel<3, true, true, false, true>
+                   : (use_lds ? transport_kernel<-1, false, true, true, true>
+                              : transport_kernel<-1, false, false, true, true>);
+  else
+    L.fn = special ? transport_kernel<3, true, true, false, false>
+                   : (use_lds ? transport_kernel<-1, false, true, true, false>
+                              : transport_kernel<-1, false, false, true, false>);
+  c->last_variant = special ? 0 : 1;
+  int per_cu = 0;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(L.fn), kBlock,
+                                                       L.shmem));
+  if (per_cu < 1) return fail(-3, "transport kernel does not fit on a compute unit");
+  if (c->blocks_per_cu > 0) per_cu = std::min(per_cu, c->blocks_per_cu);
+  // persistent blocks: no more than the slots can keep busy
+  const uint64_t max_blocks = (slots + kBlock - 1) / kBlock;
+  L.grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)per_cu * c->n_cu, max_blocks));
+  c->last_grid = L.grid;
+  return 0;
+}
+
+// parameters and launch of the set acquisition: the solids with their
+// image stacks in place of the single histograms
+int acquisition_launch(ggh_context* c, RunParams& P, Launch& L) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!c->acq_n_proj) return fail(-3, "no acquisition set");
+  if (c->acq_hist.size() != c->solids.size()) return fail(-3, "solids changed since the acquisition was set");
+  std::vector<DevSolid> sol = c->solids;
+  for (size_t k = 0; k < sol.size(); ++k)
+    if (sol[k].type == kBoxSolid) {
+      sol[k].histogram = c->acq_hist[k].p;
+      sol[k].scatter = c->acq_scat[k].p;
+    }
+  // the buffer is reused, not reallocated: a captured graph holds its address
+  if (c->acq_solids.n != sol.size() && (rc = dev_alloc(c->acq_solids, sol.size()))) return rc;
+  HIP_TRY(hipMemcpyAsync(c->acq_solids.p, sol.data(), sol.size() * sizeof(DevSolid), hipMemcpyHostToDevice,
+                         c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  P = make_params(c);
+  P.solids = c->acq_solids.p;
+  P.acq_src = c->acq_src.p;
+  P.acq_box = c->acq_box.p;
+  P.acq_layout = c->acq_layout.p;
+  P.n_proj = c->acq_n_proj;
+  P.n_box = n_solids_of(c, kBoxSolid);
+  P.n_slots = c->acq_slots;
+  return pick_launch(c, c->acq_slots, true, L);
 }
 
 }  // namespace
@@ -1379,6 +1560,7 @@ int ggh_context_destroy(ggh_context* c) {
   for (auto& b : c->mesh_bvh) b.release();
   for (auto& b : c->mesh_tris) b.release();
   c->solids_dev.release();
+  drop_acquisition(c);
   c->edep.release(); c->edep2.release(); c->hit.release(); c->ptrack.release();
   c->spectrum.release(); c->cdf.release(); c->rng.release(); c->queue.release(); c->stats.release();
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -1459,6 +1641,7 @@ int ggh_add_voxel_solid(ggh_context* c, const ggh_voxel_solid_desc* d) {
   s.border.lo = mk(d->obb.border_min[0], d->obb.border_min[1], d->obb.border_min[2]);
   s.border.hi = mk(d->obb.border_max[0], d->obb.border_max[1], d->obb.border_max[2]);
   s.type = kVoxel;
+  s.box_index = -1;
   s.mat_offset = d->material_offset;
   int max_label = 0;
   for (size_t i = 0; i < n; ++i) max_label = d->labels[i] > max_label ? d->labels[i] : max_label;
@@ -1491,6 +1674,7 @@ int ggh_add_box_solid(ggh_context* c, const ggh_box_solid_desc* d) {
   s.border.lo = mk(d->obb.border_min[0], d->obb.border_min[1], d->obb.border_min[2]);
   s.border.hi = mk(d->obb.border_max[0], d->obb.border_max[1], d->obb.border_max[2]);
   s.type = kBoxSolid;
+  s.box_index = n_solids_of(c, kBoxSolid);
   s.mat_offset = d->material_offset;
   for (int i = 0; i < 3; ++i) s.ne[i] = d->n_elements[i];
   s.box_size = mk(d->box_size[0], d->box_size[1], d->box_size[2]);
@@ -1620,6 +1804,7 @@ int ggh_add_mesh_solid(ggh_context* c, const ggh_mesh_solid_desc* d) {
   s.border.lo = mk(d->obb.border_min[0], d->obb.border_min[1], d->obb.border_min[2]);
   s.border.hi = mk(d->obb.border_max[0], d->obb.border_max[1], d->obb.border_max[2]);
   s.type = kMesh;
+  s.box_index = -1;
   s.mat_offset = d->material_offset;
   s.bvh = bvh.p;
   s.tris = tb.p;
@@ -1732,6 +1917,9 @@ int ggh_reset_tallies(ggh_context* c) {
   if (c->edep2.p) HIP_TRY(hipMemsetAsync(c->edep2.p, 0, c->edep2.n * sizeof(double), c->stream));
   if (c->hit.p) HIP_TRY(hipMemsetAsync(c->hit.p, 0, c->hit.n * sizeof(int32_t), c->stream));
   if (c->ptrack.p) HIP_TRY(hipMemsetAsync(c->ptrack.p, 0, c->ptrack.n * sizeof(int32_t), c->stream));
+  for (auto* v : {&c->acq_hist, &c->acq_scat})
+    for (auto& b : *v)
+      if (b.p) HIP_TRY(hipMemsetAsync(b.p, 0, b.n * sizeof(int32_t), c->stream));
   HIP_TRY(hipMemsetAsync(c->stats.p, 0, 8 * sizeof(unsigned long long), c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return 0;
@@ -1752,58 +1940,192 @@ int ggh_set_launch(ggh_context* c, const ggh_launch_desc* d) {
 }
 
 int ggh_run(ggh_context* c, uint64_t n) {
-  int rc = check_ctx(c);
+  int rc = check_ready(c);
   if (rc) return rc;
-  if (!c->have_physics) return fail(-3, "physics tables not set");
-  if (!c->have_source) return fail(-3, "source not set");
-  if (!c->seeded) return fail(-3, "particle slots not seeded");
-  if (c->solids.empty()) return fail(-3, "no solid registered");
-  for (size_t i = 0; i < c->solids.size(); ++i)
-    if (c->solids[i].mat_offset < 0 || c->max_material[i] >= c->n_mat)
-      return fail(-3, "solid labels reach a material without physics tables");
   if (n == 0) return 0;
-  if ((rc = sync_solids(c))) return rc;
   RunParams P = make_params(c);
   // batch layout of GGEMSSource::OrganizeParticlesInBatch (src/sources/GGEMSSource.cc:403-425)
-  uint64_t nb = (uint64_t)std::ceil((float)n / (float)GGH_MAX_SLOTS);
-  if (nb == 0) nb = 1;
-  const uint64_t base = nb == 1 ? n : n / nb;
-  const uint64_t rem = nb == 1 ? 0 : n % nb;
-  const uint64_t slots = base + (rem ? 1 : 0);
-  if (slots > GGH_MAX_SLOTS) return fail(-3, "batch larger than the slot buffer");
-  if (nb > 0xffffffffull) return fail(-3, "too many batches");
-  P.n_batches = (uint32_t)nb;
-  P.base = (uint32_t)base;
-  P.rem = (uint32_t)rem;
-  P.n_slots = (uint32_t)slots;
-  const bool single = n_solids_of(c, kVoxel) == 1 && n_solids_of(c, kMesh) == 0;
-  const size_t lds = tables_lds_bytes(c);
-  const bool use_lds = lds <= (size_t)kLdsBudget;
-  const size_t shmem = use_lds ? lds : 0;
-  // kernel variants: the specialised one (three activated processes, one
-  // voxel solid and no mesh, tables in LDS, no fluence tracking) and the
-  // general one
-  using Kernel = void (*)(RunParams);
-  const bool special = c->n_proc == 3 && single && use_lds && !c->ptrack.p && !c->force_general;
-  const Kernel fn = special ? transport_kernel<3, true, true, false>
-                            : (use_lds ? transport_kernel<-1, false, true, true> : transport_kernel<-1, false, false, true>);
-  c->last_variant = special ? 0 : 1;
-  int per_cu = 0;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn), kBlock, shmem));
-  if (per_cu < 1) return fail(-3, "transport kernel does not fit on a compute unit");
-  if (c->blocks_per_cu > 0) per_cu = std::min(per_cu, c->blocks_per_cu);
-  // persistent blocks: no more than the slots can keep busy
-  const uint64_t max_blocks = (slots + kBlock - 1) / kBlock;
-  const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)per_cu * c->n_cu, max_blocks));
-  c->last_grid = grid;
+  uint32_t nb, base, rem, slots;
+  if ((rc = batch_layout(n, nb, base, rem, slots))) return rc;
+  P.n_batches = nb;
+  P.base = base;
+  P.rem = rem;
+  P.n_slots = slots;
+  Launch L;
+  if ((rc = pick_launch(c, slots, false, L))) return rc;
   HIP_TRY(hipMemsetAsync(c->queue.p, 0, sizeof(uint32_t), c->stream));
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
-  hipLaunchKernelGGL(fn, dim3(grid), dim3(kBlock), shmem, c->stream, P);
+  hipLaunchKernelGGL(L.fn, dim3(L.grid), dim3(kBlock), L.shmem, c->stream, P);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
   HIP_TRY(hipEventSynchronize(c->ev1));
   HIP_TRY(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
   return 0;
+}
+
+int ggh_set_acquisition(ggh_context* c, const ggh_acquisition_desc* d) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  const int n_box = n_solids_of(c, kBoxSolid);
+  if (!d || d->n_projections < 1 || d->n_projections > GGH_MAX_PROJECTIONS || !d->source_matrices ||
+      !d->n_particles || (n_box > 0 && !d->box_matrices))
+    return fail(-2, "invalid acquisition description");
+  const int np = d->n_projections;
+  std::vector<uint4> lay((size_t)np);
+  uint32_t max_slots = 0;
+  for (int p = 0; p < np; ++p) {
+    uint32_t nb, base, rem, slots;
+    if ((rc = batch_layout(d->n_particles[p], nb, base, rem, slots))) return rc;
+    lay[p] = make_uint4(nb, base, rem, 0u);
+    max_slots = std::max(max_slots, slots);
+  }
+  std::vector<Mat34> src((size_t)np), box((size_t)np * n_box);
+  for (int p = 0; p < np; ++p) src[p] = to_mat34(d->source_matrices + 16 * (size_t)p);
+  for (size_t k = 0; k < box.size(); ++k) box[k] = to_mat34(d->box_matrices + 16 * k);
+  drop_acquisition(c);
+  if ((rc = dev_upload(c->acq_src, src.data(), src.size(), c->stream)) ||
+      (rc = dev_upload(c->acq_box, box.data(), box.size(), c->stream)) ||
+      (rc = dev_upload(c->acq_layout, lay.data(), lay.size(), c->stream)))
+    return rc;
+  c->acq_hist.resize(c->solids.size());
+  c->acq_scat.resize(c->solids.size());
+  for (size_t k = 0; k < c->solids.size(); ++k) {
+    if (c->solids[k].type != kBoxSolid) continue;
+    const size_t n_img = (size_t)np * c->hist_elems[k];
+    if ((rc = dev_alloc(c->acq_hist[k], n_img))) return rc;
+    HIP_TRY(hipMemsetAsync(c->acq_hist[k].p, 0, n_img * sizeof(int32_t), c->stream));
+    if (c->scat[k].p) {
+      if ((rc = dev_alloc(c->acq_scat[k], n_img))) return rc;
+      HIP_TRY(hipMemsetAsync(c->acq_scat[k].p, 0, n_img * sizeof(int32_t), c->stream));
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->acq_n_proj = np;
+  c->acq_slots = max_slots;
+  return 0;
+}
+
+int ggh_run_acquisition(ggh_context* c) {
+  RunParams P;
+  Launch L;
+  int rc = acquisition_launch(c, P, L);
+  if (rc) return rc;
+  if (P.n_slots == 0) return 0;
+  HIP_TRY(hipMemsetAsync(c->queue.p, 0, sizeof(uint32_t), c->stream));
+  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  hipLaunchKernelGGL(L.fn, dim3(L.grid), dim3(kBlock), L.shmem, c->stream, P);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  HIP_TRY(hipEventSynchronize(c->ev1));
+  HIP_TRY(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+  return 0;
+}
+
+int ggh_capture_acquisition(ggh_context* c) {
+  RunParams P;
+  Launch L;
+  int rc = acquisition_launch(c, P, L);
+  if (rc) return rc;
+  if (P.n_slots == 0) return fail(-3, "the acquisition has no primaries");
+  drop_graph(c);
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, c->stream, c->queue.p);
+  hipLaunchKernelGGL(L.fn, dim3(L.grid), dim3(kBlock), L.shmem, c->stream, P);
+  const hipError_t le = hipGetLastError();
+  hipGraph_t g = nullptr;
+  const hipError_t ce = hipStreamEndCapture(c->stream, &g);
+  if (le != hipSuccess || ce != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    const hipError_t e = le != hipSuccess ? le : ce;
+    return fail(-100 - (int)e, std::string("acquisition capture: ") + hipGetErrorString(e));
+  }
+  c->graph = g;
+  const hipError_t ie = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+  if (ie != hipSuccess) {
+    drop_graph(c);
+    return fail(-100 - (int)ie, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+  }
+  return 0;
+}
+
+int ggh_replay_acquisition(ggh_context* c) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!c->graph_exec) return fail(-3, "no captured acquisition");
+  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  HIP_TRY(hipGraphLaunch(c->graph_exec, c->stream));
+  HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  HIP_TRY(hipEventSynchronize(c->ev1));
+  HIP_TRY(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+  return 0;
+}
+
+int ggh_get_projection(ggh_context* c, int solid, int projection, int32_t* counts, int32_t* scatter) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (solid < 0 || solid >= (int)c->solids.size() || c->solids[solid].type != kBoxSolid)
+    return fail(-2, "not a box solid");
+  if (!c->acq_n_proj || projection < 0 || projection >= c->acq_n_proj) return fail(-2, "no such projection");
+  const size_t n = (size_t)c->hist_elems[solid], off = (size_t)projection * n;
+  if (counts)
+    HIP_TRY(hipMemcpyAsync(counts, c->acq_hist[solid].p + off, n * sizeof(int32_t), hipMemcpyDeviceToHost,
+                           c->stream));
+  if (scatter) {
+    if (!c->acq_scat[solid].p) return fail(-2, "scatter histogram not stored for this solid");
+    HIP_TRY(hipMemcpyAsync(scatter, c->acq_scat[solid].p + off, n * sizeof(int32_t), hipMemcpyDeviceToHost,
+                           c->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ggh_autotune(ggh_context* c, uint64_t n_trial, ggh_launch_desc* best) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (n_trial == 0) return fail(-2, "empty trial");
+  DevBuf<uint32_t> saved;
+  if ((rc = dev_alloc(saved, c->rng.n))) return rc;
+  HIP_TRY(hipMemcpyAsync(saved.p, c->rng.p, c->rng.n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+  const int keep_hot = c->hot_threshold, keep_res = c->resume_threshold, keep_batch = c->interact_batch,
+            keep_blocks = c->blocks_per_cu;
+  float best_ms = INFINITY;
+  ggh_launch_desc b{};
+  for (int blocks : {1, 2})
+    for (int hot : {24, 32, 40})
+      for (int batch : {8, 16, 32}) {
+        c->blocks_per_cu = blocks;
+        c->hot_threshold = hot;
+        c->resume_threshold = std::max(keep_res, hot);
+        c->interact_batch = batch;
+        HIP_TRY(hipMemcpyAsync(c->rng.p, saved.p, saved.n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+        if ((rc = ggh_run(c, n_trial))) break;
+        if (c->last_ms < best_ms) {
+          best_ms = c->last_ms;
+          b.hot_threshold = hot;
+          b.resume_threshold = c->resume_threshold;
+          b.interact_batch = batch;
+          b.blocks_per_cu = blocks;
+          b.general_kernel = c->force_general ? 1 : 0;
+        }
+      }
+  const hipError_t e = hipMemcpyAsync(c->rng.p, saved.p, saved.n * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                                      c->stream);
+  (void)hipStreamSynchronize(c->stream);
+  saved.release();
+  if (rc || e != hipSuccess) {
+    c->hot_threshold = keep_hot;
+    c->resume_threshold = keep_res;
+    c->interact_batch = keep_batch;
+    c->blocks_per_cu = keep_blocks;
+    return rc ? rc : fail(-100 - (int)e, "autotune: restoring the random streams failed");
+  }
+  c->hot_threshold = b.hot_threshold;
+  c->resume_threshold = b.resume_threshold;
+  c->interact_batch = b.interact_batch;
+  c->blocks_per_cu = b.blocks_per_cu;
+  if (best) *best = b;
+  return ggh_reset_tallies(c);
 }
 
 int ggh_synchronize(ggh_context* c) {

@@ -53,6 +53,26 @@ void run_ggems(GGEMS* g) {
   API_TRY g->run();
   API_CATCH()
 }
+void run_acquisition_ggems(GGEMS* g, size_t n, const float* rx, const float* ry, const float* rz, char const* unit,
+                           const uint64_t* n_particles) {
+  API_TRY if (!rx || !ry || !rz || !n_particles) raise("GGEMS", "null acquisition arguments");
+  std::vector<float> a(3 * n);
+  for (size_t p = 0; p < n; ++p) {
+    a[p] = angle_unit(rx[p], str(unit));
+    a[n + p] = angle_unit(ry[p], str(unit));
+    a[2 * n + p] = angle_unit(rz[p], str(unit));
+  }
+  g->run_acquisition(n, a.data(), a.data() + n, a.data() + 2 * n, n_particles);
+  API_CATCH()
+}
+void repeat_acquisition_ggems(GGEMS* g) {
+  API_TRY g->repeat_acquisition();
+  API_CATCH()
+}
+void autotune_ggems(GGEMS* g, uint64_t n_trial) {
+  API_TRY g->autotune(n_trial);
+  API_CATCH()
+}
 
 // ---- OpenCL manager (HIP device selection) --------------------------------------------
 GGEMSOpenCLManager* get_instance_ggems_opencl_manager(void) { return &GGEMSOpenCLManager::get(); }

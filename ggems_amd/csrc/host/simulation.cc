@@ -483,6 +483,34 @@ void GGEMSCTSystem::save_results(ggh_context* ctx) {
   }
 }
 
+// the projections of an acquisition, one MHD stack per CT system (same
+// module layout as SaveResults, one image per projection)
+void GGEMSCTSystem::save_acquisition(ggh_context* ctx, int n_proj) {
+  const size_t ex = elements[0], ey = elements[1];
+  const int dim[3] = {(int)(modules[0] * ex), (int)(modules[1] * ey), n_proj};
+  const size_t plane = (size_t)dim[0] * dim[1];
+  std::vector<int32_t> out(plane * n_proj, 0), sout(store_scatter ? plane * n_proj : 0, 0), h(ex * ey), s(ex * ey);
+  for (int p = 0; p < n_proj; ++p)
+    for (size_t jj = 0; jj < modules[1]; ++jj)
+      for (size_t ii = 0; ii < modules[0]; ++ii) {
+        int k = solid_indices[ii + jj * modules[0]];
+        engine_check(ggh_get_projection(ctx, k, p, h.data(), store_scatter ? s.data() : nullptr), "read projection");
+        for (size_t jjj = 0; jjj < ey; ++jjj)
+          for (size_t iii = 0; iii < ex; ++iii) {
+            size_t o = (size_t)p * plane + (iii + ii * ex) + (jjj + jj * ey) * (size_t)dim[0];
+            out[o] = h[iii + jjj * ex];
+            if (store_scatter) sout[o] = s[iii + jjj * ex];
+          }
+      }
+  write_mhd(output_basename, "MET_INT", dim, element_size, out.data(), sizeof(int32_t));
+  if (store_scatter) {
+    std::string sname = output_basename;
+    size_t f = sname.find(".mhd");
+    sname = f == std::string::npos ? sname + "-scatter.mhd" : sname.substr(0, f) + "-scatter.mhd";
+    write_mhd(sname, "MET_INT", dim, element_size, sout.data(), sizeof(int32_t));
+  }
+}
+
 // ---------------------------------------------------------------------------
 // dosimetry (src/navigators/GGEMSDosimetryCalculator.cc:435-809)
 // ---------------------------------------------------------------------------
@@ -758,4 +786,65 @@ void GGEMS::run() {
   last_run_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
   for (GGEMSNavigator* n : GGEMSNavigatorManager::get().navigators) n->save_results(ctx);
   log(0, "GGEMS", "GGEMS simulation succeeded");
+}
+
+void GGEMS::run_acquisition(size_t n, const float* rx, const float* ry, const float* rz, const uint64_t* np) {
+  if (!initialized) raise("GGEMS", "GGEMS is not initialized!!!");
+  auto& sources = GGEMSSourceManager::get().sources;
+  if (sources.size() != 1) raise("GGEMS", "A CT acquisition needs exactly one X-ray source!!!");
+  if (n == 0 || n > GGH_MAX_PROJECTIONS) raise("GGEMS", "Invalid number of projections!!!");
+  // module placements in engine registration order (CT systems in navigator order)
+  std::vector<const Transform*> modules;
+  for (GGEMSNavigator* nav : GGEMSNavigatorManager::get().navigators)
+    if (auto* ct = dynamic_cast<GGEMSCTSystem*>(nav))
+      for (const Transform& t : ct->module_transforms) modules.push_back(&t);
+  std::vector<float> src_m(16 * n), box_m(16 * n * modules.size());
+  for (size_t p = 0; p < n; ++p) {
+    Transform t = sources.front()->transform;
+    t.rotate(rx[p], ry[p], rz[p]);
+    std::memcpy(&src_m[16 * p], t.m.data(), sizeof(float) * 16);
+    for (size_t j = 0; j < modules.size(); ++j) {
+      Transform u = *modules[j];
+      u.rotate(rx[p], ry[p], rz[p]);
+      std::memcpy(&box_m[16 * (p * modules.size() + j)], u.m.data(), sizeof(float) * 16);
+    }
+  }
+  upload_source(*sources.front());
+  ggh_acquisition_desc d;
+  d.n_projections = (int32_t)n;
+  d.source_matrices = src_m.data();
+  d.box_matrices = box_m.data();
+  d.n_particles = np;
+  engine_check(ggh_set_acquisition(ctx, &d), "set acquisition");
+  acquisition_projections = (int)n;
+  engine_check(ggh_capture_acquisition(ctx), "capture acquisition");
+  repeat_acquisition();
+}
+
+void GGEMS::repeat_acquisition() {
+  if (!acquisition_projections) raise("GGEMS", "No CT acquisition to repeat!!!");
+  auto t0 = std::chrono::steady_clock::now();
+  engine_check(ggh_replay_acquisition(ctx), "acquisition");
+  auto t1 = std::chrono::steady_clock::now();
+  last_run_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  save_acquisition();
+  log(0, "GGEMS", "GGEMS acquisition succeeded");
+}
+
+void GGEMS::save_acquisition() {
+  for (GGEMSNavigator* nav : GGEMSNavigatorManager::get().navigators) {
+    if (auto* ct = dynamic_cast<GGEMSCTSystem*>(nav)) ct->save_acquisition(ctx, acquisition_projections);
+    else if (nav->dosimetry) nav->dosimetry->save(ctx);
+  }
+}
+
+void GGEMS::autotune(uint64_t n_trial) {
+  if (!initialized) raise("GGEMS", "GGEMS is not initialized!!!");
+  auto& sources = GGEMSSourceManager::get().sources;
+  if (sources.empty()) raise("GGEMS", "No source registered!!!");
+  upload_source(*sources.front());
+  ggh_launch_desc best;
+  engine_check(ggh_autotune(ctx, n_trial, &best), "autotune");
+  log(1, "GGEMS", "autotuned launch: hot threshold " + std::to_string(best.hot_threshold) + ", interaction batch " +
+                      std::to_string(best.interact_batch) + ", blocks per CU " + std::to_string(best.blocks_per_cu));
 }

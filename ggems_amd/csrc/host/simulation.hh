@@ -119,6 +119,7 @@ struct GGEMSCTSystem : GGEMSNavigator {
   void initialize() override;
   int register_solids(ggh_context* ctx, int material_offset) override;
   void save_results(ggh_context* ctx) override;
+  void save_acquisition(ggh_context* ctx, int n_projections);  // image stacks [n_projections][rows][cols]
 };
 
 // GGEMSDosimetryCalculator (src/navigators/GGEMSDosimetryCalculator.cc)
@@ -212,6 +213,14 @@ struct GGEMS {
   void initialize(uint32_t seed);
   void run();
   void upload_source(const GGEMSXRaySource& src);
+  // CT acquisition (one launch, captured as a hipGraph): projection p turns
+  // the source and every CT system by rotation[p] (radians) on top of their
+  // placement and transports n_particles[p] primaries
+  void run_acquisition(size_t n, const float* rx, const float* ry, const float* rz, const uint64_t* n_particles);
+  void repeat_acquisition();  // replays the captured acquisition (more primaries, same placements)
+  void autotune(uint64_t n_trial);
+  void save_acquisition();
+  int acquisition_projections = 0;
 };
 
 // the engine context of the most recently initialised GGEMS object

@@ -69,6 +69,11 @@ class LaunchDesc(C.Structure):
                 ("blocks_per_cu", C.c_int32), ("general_kernel", C.c_int32)]
 
 
+class AcquisitionDesc(C.Structure):
+    _fields_ = [("n_projections", C.c_int32), ("source_matrices", C.c_void_p), ("box_matrices", C.c_void_p),
+                ("n_particles", C.c_void_p)]
+
+
 class Stats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("primaries", "voxel_steps", "interactions", "detector_steps",
                                           "detector_interactions", "hops", "mesh_steps", "mesh_interactions")]
@@ -117,6 +122,12 @@ def library():
             "ggh_debug_neg_log": ([vp, C.c_int, vp, vp, vp], C.c_int),
             "ggh_add_mesh_solid": ([vp, C.POINTER(MeshSolidDesc)], C.c_int),
             "ggh_debug_mesh_hits": ([vp, C.c_int, C.c_int, vp, vp, vp, vp], C.c_int),
+            "ggh_set_acquisition": ([vp, C.POINTER(AcquisitionDesc)], C.c_int),
+            "ggh_run_acquisition": ([vp], C.c_int),
+            "ggh_get_projection": ([vp, C.c_int, C.c_int, vp, vp], C.c_int),
+            "ggh_capture_acquisition": ([vp], C.c_int),
+            "ggh_replay_acquisition": ([vp], C.c_int),
+            "ggh_autotune": ([vp, C.c_uint64, C.POINTER(LaunchDesc)], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -393,6 +404,44 @@ class Engine:
         ref = np.empty(sums.shape, np.float32)
         _check(library().ggh_debug_neg_log(self.h, sums.size, _ptr(sums), _ptr(fast), _ptr(ref)))
         return fast, ref
+
+    # ---- CT acquisition ---------------------------------------------------
+    def set_acquisition(self, source_matrices, box_matrices, n_particles):
+        """source_matrices [P][4][4], box_matrices [P][n_box][4][4] (box solids
+        in registration order), n_particles [P]: one persistent launch per
+        run_acquisition, equal to P consecutive runs with those placements."""
+        src = np.ascontiguousarray(source_matrices, np.float32).reshape(-1, 16)
+        n_proj = src.shape[0]
+        box = np.ascontiguousarray(box_matrices, np.float32).reshape(n_proj, -1)
+        n = np.ascontiguousarray(n_particles, np.uint64).reshape(-1)
+        if n.size != n_proj:
+            raise EngineError("one photon budget per projection")
+        self._acq = (src, box, n)
+        d = AcquisitionDesc(n_proj, _ptr(src).value, _ptr(box).value if box.size else None, _ptr(n).value)
+        _check(library().ggh_set_acquisition(self.h, C.byref(d)))
+        self.n_projections = n_proj
+
+    def run_acquisition(self):
+        _check(library().ggh_run_acquisition(self.h))
+
+    def capture_acquisition(self):
+        _check(library().ggh_capture_acquisition(self.h))
+
+    def replay_acquisition(self):
+        _check(library().ggh_replay_acquisition(self.h))
+
+    def projection(self, solid, p):
+        ny, nx, has_scatter = self.box_elements[solid]
+        counts = np.zeros((ny, nx), np.int32)
+        scatter = np.zeros((ny, nx), np.int32) if has_scatter else None
+        _check(library().ggh_get_projection(self.h, solid, p, _ptr(counts), _ptr(scatter)))
+        return counts, scatter
+
+    def autotune(self, n_trial):
+        """fastest launch shape over a trial run; call after seed() and before any run"""
+        d = LaunchDesc()
+        _check(library().ggh_autotune(self.h, int(n_trial), C.byref(d)))
+        return {k: int(getattr(d, k)) for k, _ in LaunchDesc._fields_}
 
     def debug_uniforms(self, n_slots, n):
         out = np.zeros((n_slots, n), np.float32)

@@ -442,6 +442,27 @@ class GGEMS:
     def run(self):
         _call("run_ggems", None, [VP], self.obj)
 
+    def run_acquisition(self, rotations, n_particles, unit="deg"):
+        """CT acquisition in one launch: projection p turns the source and every
+        CT system by rotations[p] = (rx, ry, rz) and transports n_particles[p]
+        primaries; each CT system saves a stack of projections."""
+        import numpy as np
+        r = np.ascontiguousarray(rotations, np.float32).reshape(-1, 3)
+        n = np.ascontiguousarray(n_particles, np.uint64).reshape(-1)
+        if n.size != r.shape[0]:
+            raise GGEMSError("one photon budget per projection")
+        rx, ry, rz = (np.ascontiguousarray(r[:, i]) for i in range(3))
+        fp = C.POINTER(C.c_float)
+        _call("run_acquisition_ggems", None, [VP, C.c_size_t, fp, fp, fp, C.c_char_p, C.POINTER(C.c_uint64)],
+              self.obj, r.shape[0], rx.ctypes.data_as(fp), ry.ctypes.data_as(fp), rz.ctypes.data_as(fp),
+              unit.encode(), n.ctypes.data_as(C.POINTER(C.c_uint64)))
+
+    def repeat_acquisition(self):
+        _call("repeat_acquisition_ggems", None, [VP], self.obj)
+
+    def autotune(self, n_trial):
+        _call("autotune_ggems", None, [VP, C.c_uint64], self.obj, int(n_trial))
+
     def opencl_verbose(self, flag):
         _call("set_opencl_verbose_ggems", None, [VP, B], self.obj, flag)
 

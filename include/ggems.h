@@ -227,6 +227,20 @@ void attach_to_navigator_dosimetry_calculator(GGEMSDosimetryCalculator* dose_cal
 /* ---- additions with no counterpart in GGEMS (tests, benchmarks, tools) ----- */
 /* message of the most recent failed call on this thread not yet read ("" if none);
  * reading it clears it, so call it after each call whose success matters */
+/* CT acquisition in one launch (captured as a hipGraph): projection p turns
+ * the X-ray source and every CT system by (rx[p], ry[p], rz[p]) [unit] on
+ * top of their placement - the matrices SetRotation would give - and
+ * transports n_particles[p] primaries; each CT system saves a stack of n
+ * projections at its save() basename (DimSize cols rows n).  The images
+ * equal those of n consecutive runs with those placements. */
+void run_acquisition_ggems(GGEMS* ggems, size_t n, const float* rx, const float* ry, const float* rz,
+                           char const* unit, const uint64_t* n_particles);
+/* replays the last acquisition (its random streams continue) and saves the accumulated stacks */
+void repeat_acquisition_ggems(GGEMS* ggems);
+/* times trial runs of n_trial primaries over launch shapes and keeps the
+ * fastest; call after initialize_ggems and before any run (it restores the
+ * seeded streams and zeroes every tally) */
+void autotune_ggems(GGEMS* ggems, uint64_t n_trial);
 char const* get_last_error_ggems(void);
 /* deletes every registered navigator and source and restores the process-table defaults */
 void clean_safely_ggems(void);

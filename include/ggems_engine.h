@@ -180,6 +180,43 @@ int ggh_set_launch(ggh_context* ctx, const ggh_launch_desc* desc);
 int ggh_last_grid(ggh_context* ctx, int32_t* blocks, int32_t* block_size, int32_t* variant);
 
 int ggh_get_histogram(ggh_context* ctx, int solid, int32_t* counts, int32_t* scatter);
+
+/* ---- CT acquisition ------------------------------------------------------
+ * A gantry of n_projections positions transported in ONE persistent launch.
+ * The reference runs one projection per GGEMS::Run after placing the source
+ * and the CT system with SetRotation (src/navigators/GGEMSCTSystem.cc:
+ * 279-288, src/global/GGEMS.cc:303-363); here projection p places the
+ * source at source_matrices[p] and box solid j (registration order among
+ * the box solids) at box_matrices[p][j], and transports n_particles[p]
+ * primaries laid out in batches as ggh_run lays them out.  Slot s runs its
+ * primaries of projection 0, then 1, ..., so its random stream continues
+ * exactly as over consecutive ggh_run calls: the images equal those of
+ * n_projections ggh_run calls with the same placements.  Each box solid
+ * scores n_projections images; phantom dose tallies sum over projections. */
+#define GGH_MAX_PROJECTIONS 65536
+typedef struct {
+  int32_t n_projections;
+  const float* source_matrices;  /* [n_projections][16] row-major local->global */
+  const float* box_matrices;     /* [n_projections][n_box_solids][16] */
+  const uint64_t* n_particles;   /* [n_projections] */
+} ggh_acquisition_desc;
+/* uploads placements and zeroes the images (tallies of earlier acquisitions are dropped) */
+int ggh_set_acquisition(ggh_context* ctx, const ggh_acquisition_desc* desc);
+/* one launch of the set acquisition; images and dose tallies accumulate */
+int ggh_run_acquisition(ggh_context* ctx);
+/* image `projection` of box solid `solid` (counts and optionally scatter) */
+int ggh_get_projection(ggh_context* ctx, int solid, int projection, int32_t* counts, int32_t* scatter);
+/* the set acquisition captured as a hipGraph (slot-queue reset + transport
+ * launch); replays continue the random streams like further
+ * ggh_run_acquisition calls.  ggh_set_acquisition drops the graph. */
+int ggh_capture_acquisition(ggh_context* ctx);
+int ggh_replay_acquisition(ggh_context* ctx);
+
+/* Launch autotuner: times ggh_run(n_trial) over a grid of launch shapes
+ * (blocks per compute unit, hot threshold, interaction batch) and keeps the
+ * fastest.  Call after ggh_seed and before any run: it restores the seeded
+ * random streams and zeroes every tally and counter afterwards. */
+int ggh_autotune(ggh_context* ctx, uint64_t n_trial, ggh_launch_desc* best);
 int ggh_get_dose_tallies(ggh_context* ctx, double* edep, double* edep_squared, int32_t* hit,
                          int32_t* photon_tracking);
 int ggh_compute_dose(ggh_context* ctx, float scale_factor, int32_t water_reference, float minimum_density,

@@ -168,6 +168,17 @@ uint64_t ggo_run_slots(const ggo_physics* ph, const ggo_solid* solids, int n_sol
                        const ggo_source* src, uint64_t n_particles, uint32_t seed,
                        uint64_t slot_begin, uint64_t slot_end, int threads, ggo_stats* stats);
 
+/* CT acquisition as consecutive runs: projection p places the source at
+ * src_mats[p] and box solid j (rank among the box solids) at
+ * box_mats[p][j] (row-major 4x4) and transports n_particles[p] primaries in
+ * the reference's batch layout; slot s runs its primaries of projection 0,
+ * 1, ... in order, continuing its JKISS state as consecutive GGEMS::Run
+ * calls would (src/global/GGEMS.cc:303-363).  The histogram/scatter of
+ * every box solid points at n_projections consecutive images. */
+int ggo_run_acquisition(const ggo_physics* ph, const ggo_solid* solids, int n_solids, const ggo_source* src,
+                        int n_projections, const float* src_mats, const float* box_mats,
+                        const uint64_t* n_particles, uint32_t seed, int threads, ggo_stats* stats);
+
 /* number of batches / slots for n particles (reference batch split) */
 void ggo_batch_layout(uint64_t n_particles, uint64_t* n_batches, uint64_t* n_slots);
 

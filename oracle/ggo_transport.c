@@ -836,6 +836,122 @@ int ggo_run(const ggo_physics* ph, const ggo_solid* solids, int ns, const ggo_so
 }
 
 /* ------------------------------------------------------------------------ */
+/* CT acquisition: consecutive runs, one per gantry position                */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  const ggo_physics* ph;
+  const ggo_solid* solids;
+  int ns;
+  const ggo_source* src;
+  int np;
+  const float* src_mats;
+  const float* box_mats;
+  int n_box;
+  const uint64_t* lay; /* [np][3]: n_batches, base, rem */
+  uint32_t* states;
+  uint64_t n_slots;
+  uint64_t s0, s1;
+  ggo_stats st;
+} acq_worker;
+
+static void* acq_run(void* arg) {
+  acq_worker* w = (acq_worker*)arg;
+  memset(&w->st, 0, sizeof w->st);
+  ggo_solid* sol = (ggo_solid*)malloc((size_t)w->ns * sizeof(ggo_solid));
+  if (!sol) return NULL;
+  memcpy(sol, w->solids, (size_t)w->ns * sizeof(ggo_solid));
+  ggo_source src = *w->src;
+  for (uint64_t s = w->s0; s < w->s1; ++s) {
+    uint32_t rng[5];
+    for (int k = 0; k < 5; ++k) rng[k] = w->states[k * w->n_slots + s];
+    for (int p = 0; p < w->np; ++p) {
+      const uint64_t nb = w->lay[3 * p], base = w->lay[3 * p + 1], rem = w->lay[3 * p + 2];
+      /* photon s of each batch: batch b holds base + (b < rem) photons */
+      uint64_t todo = s < base ? nb : (s == base ? rem : 0);
+      if (!todo) continue;
+      memcpy(src.m, w->src_mats + 16 * (size_t)p, sizeof src.m);
+      for (int k = 0, j = 0; k < w->ns; ++k) {
+        if (w->solids[k].type != GGO_SOLID_BOX) continue;
+        const size_t img = (size_t)p * w->solids[k].nelem[0] * w->solids[k].nelem[1];
+        memcpy(sol[k].obb.m, w->box_mats + 16 * ((size_t)p * w->n_box + j), sizeof sol[k].obb.m);
+        sol[k].histogram = w->solids[k].histogram + img;
+        sol[k].scatter = w->solids[k].scatter ? w->solids[k].scatter + img : NULL;
+        ++j;
+      }
+      while (todo--) transport_primary(w->ph, sol, w->ns, &src, rng, &w->st);
+    }
+    for (int k = 0; k < 5; ++k) w->states[k * w->n_slots + s] = rng[k];
+  }
+  free(sol);
+  return NULL;
+}
+
+int ggo_run_acquisition(const ggo_physics* ph, const ggo_solid* solids, int ns, const ggo_source* src, int np,
+                        const float* src_mats, const float* box_mats, const uint64_t* n_particles, uint32_t seed,
+                        int threads, ggo_stats* stats) {
+  if (rayleigh_init() || np < 1) return -1;
+  uint64_t* lay = (uint64_t*)malloc(3 * (size_t)np * sizeof(uint64_t));
+  uint32_t* states = (uint32_t*)malloc(5 * MAX_SLOTS * sizeof(uint32_t));
+  if (!lay || !states) {
+    free(lay);
+    free(states);
+    return -1;
+  }
+  uint64_t max_slots = 0;
+  for (int p = 0; p < np; ++p) {
+    uint64_t nb, nslots, n = n_particles[p];
+    ggo_batch_layout(n, &nb, &nslots);
+    lay[3 * p] = nb;
+    lay[3 * p + 1] = nb == 1 ? n : n / nb;
+    lay[3 * p + 2] = nb == 1 ? 0 : n % nb;
+    if (nslots > max_slots) max_slots = nslots;
+  }
+  int n_box = 0;
+  for (int k = 0; k < ns; ++k) n_box += solids[k].type == GGO_SOLID_BOX;
+  ggo_seed_slots(seed, MAX_SLOTS, states);
+  if (threads <= 0) threads = (int)sysconf(_SC_NPROCESSORS_ONLN);
+  if (threads < 1) threads = 1;
+  acq_worker* ws = (acq_worker*)calloc((size_t)threads, sizeof(acq_worker));
+  pthread_t* th = (pthread_t*)calloc((size_t)threads, sizeof(pthread_t));
+  for (int t = 0; t < threads; ++t) {
+    acq_worker* w = &ws[t];
+    w->ph = ph;
+    w->solids = solids;
+    w->ns = ns;
+    w->src = src;
+    w->np = np;
+    w->src_mats = src_mats;
+    w->box_mats = box_mats;
+    w->n_box = n_box;
+    w->lay = lay;
+    w->states = states;
+    w->n_slots = MAX_SLOTS;
+    w->s0 = max_slots * (uint64_t)t / (uint64_t)threads;
+    w->s1 = max_slots * (uint64_t)(t + 1) / (uint64_t)threads;
+    pthread_create(&th[t], NULL, acq_run, w);
+  }
+  ggo_stats tot;
+  memset(&tot, 0, sizeof tot);
+  for (int t = 0; t < threads; ++t) {
+    pthread_join(th[t], NULL);
+    tot.primaries += ws[t].st.primaries;
+    tot.voxel_steps += ws[t].st.voxel_steps;
+    tot.interactions += ws[t].st.interactions;
+    tot.detector_steps += ws[t].st.detector_steps;
+    tot.detector_interactions += ws[t].st.detector_interactions;
+    tot.hops += ws[t].st.hops;
+    tot.mesh_steps += ws[t].st.mesh_steps;
+    tot.mesh_interactions += ws[t].st.mesh_interactions;
+  }
+  if (stats) *stats = tot;
+  free(ws);
+  free(th);
+  free(states);
+  free(lay);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* test entry points                                                        */
 /* ------------------------------------------------------------------------ */
 int ggo_voxel_walk(const ggo_solid* s, const float* p, const float* d, int32_t* out, int max_out) {

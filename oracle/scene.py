@@ -200,7 +200,7 @@ class Scene:
         return self
 
     # ---- oracle ---------------------------------------------------------------
-    def _oracle_structs(self, tallies):
+    def _oracle_structs(self, tallies, images=None):
         ph = O.Physics()
         ph.nbins = self.nbins
         ph.energy_bins = O.ptr(self.ebins)
@@ -250,7 +250,8 @@ class Scene:
                     s.nelem[:] = mod["ne"].tolist()
                     s.box_size[:] = mod["size"].tolist()
                     s.threshold = pay["threshold"]
-                    h = np.zeros((mod["ne"][1], mod["ne"][0]), np.int32)
+                    shape = (mod["ne"][1], mod["ne"][0]) if images is None else (images, mod["ne"][1], mod["ne"][0])
+                    h = np.zeros(shape, np.int32)
                     tallies.setdefault("hist", []).append(h)
                     s.histogram = O.ptr(h)
                     if pay["store_scatter"]:
@@ -319,6 +320,56 @@ class Scene:
                                   threads, C.byref(st))
         tallies["stats"] = {k: int(getattr(st, k)) for k, _ in O.Stats._fields_}
         return tallies
+
+    # ---- CT acquisition ---------------------------------------------------------
+    def acquisition_matrices(self, rotations):
+        """Per projection, the source and every CT module turned by
+        R = Rz*Ry*Rx of rotations[p] (radians) on top of the scene's
+        placement: R left-multiplies the matrix, as SetRotation does
+        (src/maths/GGEMSGeometryTransformation.cc:196-269)."""
+        src, box = [], []
+        mods = [mod for kind, _, pay in self.navigators if kind == "ct" for mod in pay["modules"]]
+        for r in rotations:
+            R = Transform().rotate(r).m
+            src.append(mat_mult(R, self.source["m"]))
+            box.append([mat_mult(R, mod["m"]) for mod in mods])
+        return np.array(src, F).reshape(len(rotations), 4, 4), np.array(box, F).reshape(len(rotations), len(mods), 4, 4)
+
+    def run_oracle_acquisition(self, rotations, n_particles, seed, threads=0):
+        """The projections as consecutive runs (slot-major, continuing JKISS
+        states); hist/scatter are per-module stacks [P][rows][cols]."""
+        src, box = self.acquisition_matrices(rotations)
+        tallies = {}
+        ph, solids, source = self._oracle_structs(tallies, images=len(rotations))
+        n = np.ascontiguousarray(n_particles, np.uint64)
+        st = O.Stats()
+        rc = O.lib().ggo_run_acquisition(C.byref(ph), solids, len(solids), C.byref(source), len(rotations),
+                                         O.ptr(np.ascontiguousarray(src)), O.ptr(np.ascontiguousarray(box)),
+                                         O.ptr(n), seed, threads, C.byref(st))
+        if rc != 0:
+            raise RuntimeError("oracle acquisition failed")
+        tallies["stats"] = {k: int(getattr(st, k)) for k, _ in O.Stats._fields_}
+        return tallies
+
+    def engine_acquisition(self, rotations, n_particles, seed, eng=None):
+        """An engine with the acquisition set (seeded, tallies zero)."""
+        eng = eng or self.make_engine()
+        src, box = self.acquisition_matrices(rotations)
+        eng.seed(seed)
+        eng.reset_tallies()
+        eng.set_acquisition(src, box, n_particles)
+        return eng
+
+    def read_acquisition(self, eng):
+        out = {"hist": [], "scatter": []}
+        for k in self.engine_solids.get("box", []):
+            hs = [eng.projection(k, p) for p in range(eng.n_projections)]
+            out["hist"].append(np.stack([h for h, _ in hs]))
+            if hs[0][1] is not None:
+                out["scatter"].append(np.stack([s for _, s in hs]))
+        out["stats"] = eng.stats()
+        out["kernel_ms"] = eng.last_kernel_ms()
+        return out
 
     # ---- engine ---------------------------------------------------------------
     def make_engine(self, device=0):

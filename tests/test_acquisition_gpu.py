@@ -1,0 +1,143 @@
+"""CT acquisition in one persistent launch on the GPU: against the oracle's
+consecutive runs, bit-exact against consecutive single-projection launches,
+hipGraph replays against eager launches, the autotuner's state restore, and
+the GGEMS-level script."""
+
+import math
+import os
+
+import numpy as np
+import pytest
+
+from ggems_amd import authoring, configs
+from scene import Scene, cfg1_scene
+
+pytestmark = pytest.mark.gpu
+
+ROT = [(0.0, 0.0, 0.0), (0.0, 0.0, 0.4), (0.0, 0.0, 1.3), (0.2, 0.0, 2.0)]
+BUDGET = [150_000, 0, 1_300_000, 40_000]  # an empty projection and a two-batch one
+
+
+def _scene():
+    return cfg1_scene(n_side=60, detector=48, dosimetry=True, aperture_deg=2.0)
+
+
+def _images(sc, eng):
+    out = sc.read_acquisition(eng)
+    return out["hist"][0], out["scatter"][0], out["stats"]
+
+
+def test_acquisition_matches_oracle():
+    sc = _scene()
+    eng = sc.engine_acquisition(ROT, BUDGET, 17)
+    eng.run_acquisition()
+    hist, scat, stats = _images(sc, eng)
+    ref = sc.run_oracle_acquisition(ROT, BUDGET, 17)
+    assert stats["primaries"] == ref["stats"]["primaries"] == sum(BUDGET)
+    assert not hist[1].any()
+    for p in (0, 2, 3):
+        h_ref = ref["hist"][0][p]
+        l1 = np.abs(hist[p].astype(np.int64) - h_ref).sum()
+        assert l1 <= 0.02 * h_ref.sum() + 20, (p, l1, h_ref.sum())
+    assert abs(stats["voxel_steps"] - ref["stats"]["voxel_steps"]) <= 0.01 * ref["stats"]["voxel_steps"]
+    edep, _, _, _ = eng.dose_tallies()
+    assert abs(edep.sum() - ref["edep"].sum()) <= 0.01 * ref["edep"].sum()
+    eng.close()
+
+
+def test_acquisition_equals_consecutive_launches():
+    """one launch over all projections == one launch per projection, with the
+    random streams continuing: identical integer images and counters"""
+    sc = _scene()
+    eng = sc.engine_acquisition(ROT, BUDGET, 23)
+    eng.run_acquisition()
+    hist, scat, stats = _images(sc, eng)
+    eng2 = sc.make_engine()
+    eng2.seed(23)
+    eng2.reset_tallies()
+    src, box = sc.acquisition_matrices(ROT)
+    for p in range(len(ROT)):
+        eng2.set_acquisition(src[p:p + 1], box[p:p + 1], BUDGET[p:p + 1])
+        eng2.run_acquisition()
+        h, s = eng2.projection(sc.engine_solids["box"][0], 0)
+        assert np.array_equal(h, hist[p]), p
+        assert np.array_equal(s, scat[p]), p
+    assert eng2.stats() == stats
+    e1, _, hit1, _ = eng.dose_tallies()
+    e2, _, hit2, _ = eng2.dose_tallies()
+    assert np.array_equal(hit1, hit2)
+    assert np.allclose(e1, e2, rtol=1e-12, atol=0.0)
+    eng.close()
+    eng2.close()
+
+
+def test_graph_replays_match_eager_launches():
+    """each replay of the captured acquisition equals one more eager launch,
+    with eager work and synchronisations between the replays"""
+    sc = _scene()
+    eager = sc.engine_acquisition(ROT, BUDGET, 31)
+    graph = sc.engine_acquisition(ROT, BUDGET, 31)
+    graph.capture_acquisition()
+    for rep in range(3):
+        eager.run_acquisition()
+        graph.replay_acquisition()
+        graph.stats()  # eager work between replays
+        h_e, s_e, st_e = _images(sc, eager)
+        h_g, s_g, st_g = _images(sc, graph)
+        assert np.array_equal(h_e, h_g), rep
+        assert np.array_equal(s_e, s_g), rep
+        assert st_e == st_g, rep
+        assert st_g["primaries"] == (rep + 1) * sum(BUDGET)
+    eager.close()
+    graph.close()
+
+
+def test_autotune_restores_streams_and_tallies():
+    sc = _scene()
+    tuned = sc.make_engine()
+    tuned.seed(41)
+    best = tuned.autotune(2_000_000)
+    assert best["blocks_per_cu"] in (1, 2) and best["hot_threshold"] in (24, 32, 40)
+    assert best["interact_batch"] in (8, 16, 32)
+    assert tuned.stats()["primaries"] == 0
+    tuned.reset_tallies()
+    tuned.run(300_000)
+    h_t, _ = tuned.histogram(sc.engine_solids["box"][0])
+    fresh = sc.make_engine()
+    fresh.seed(41)
+    fresh.reset_tallies()
+    fresh.run(300_000)
+    h_f, _ = fresh.histogram(sc.engine_solids["box"][0])
+    assert np.array_equal(h_t, h_f)
+    tuned.close()
+    fresh.close()
+
+
+def test_acquisition_script_matches_oracle(tmp_path):
+    """configs.acquisition through the GGEMS API: the saved projection stack
+    against the oracle's consecutive runs on the same inputs"""
+    wd = str(tmp_path)
+    keep = configs.acquisition(wd, side=64, seed=3)
+    rot, budgets = configs.acquisition_plan(n_angles=3, budget_range=(100_000, 300_000), seed=2, step_deg=40.0)
+    from ggems_amd.ggems import GGEMS
+    g = GGEMS()
+    g.initialize(123)
+    g.run_acquisition(rot, budgets, "deg")
+    stack, spacing = authoring.read_mhd(os.path.join(wd, "projections.mhd"))
+    assert stack.shape == (3, 512, 384)
+    mhd, rng, spec = configs.acquisition_inputs(wd, 64, 3)
+    lab, _ = authoring.read_mhd(mhd)
+    e, w = np.loadtxt(spec, unpack=True)
+    sc = Scene()
+    sc.add_phantom(lab, authoring.BODY_MATERIALS, (1.0, 1.0, 1.0))
+    sc.add_ct_flat("GOS", (1, 1), (384, 512, 1), (0.78, 0.78, 1.0), 600.0, 1000.0, 10e-3)
+    half_diag = 0.5 * 0.78 * math.hypot(384, 512)
+    sc.set_source((-600.0, 0.0, 0.0), math.degrees(math.atan(half_diag / 1000.0)), e, intensities=w)
+    sc.build()
+    ref = sc.run_oracle_acquisition(np.radians(rot.astype(np.float64)).astype(np.float32), budgets, 123)
+    for p in range(3):
+        h_ref = ref["hist"][0][p]
+        l1 = np.abs(stack[p].astype(np.int64) - h_ref).sum()
+        assert l1 <= 0.03 * h_ref.sum() + 20, (p, l1, h_ref.sum())
+    g.delete()
+    del keep
