@@ -11,6 +11,12 @@ resident in HBM: one step transports --photons-per-step primaries
 (batches of 1,048,576 particle slots, as GGEMS runs them).
 
 Prints one JSON line (metric, value, roofline, cpu_baseline, ...).
+
+--workload mesh_ct measures the meshed-phantom scene (two icosphere meshes
+with a curved 888x64 detector, 80 keV) and --workload acquisition the CT
+acquisition in one launch (360 projections around each of the 128^3, 256^3
+and 384^3 ellipsoid phantoms, budgets scaled by --budget-scale); both print
+their own JSON line.
 """
 
 import argparse
@@ -94,6 +100,80 @@ def profile_record():
         return None
 
 
+def bench_mesh_ct(args):
+    """photons/s through the two icosphere meshes (no enclosing box: GGEMS
+    would let the box claim every photon) and the curved detector"""
+    import torch
+
+    from ggems_amd import configs
+    from ggems_amd.ggems import GGEMS
+    wd = os.path.join(REPO, "ggems_amd", "data", "_gen", "mesh_ct")
+    keep = configs.mesh_ct(wd, n_particles=args.photons_per_step, water_box=False, dosimetry=True)
+    g = GGEMS()
+    g.initialize(args.seed)
+    eng = g.engine()
+    for _ in range(args.warmup):
+        eng.run(args.photons_per_step)
+    eng.reset_tallies()
+    torch.cuda.synchronize()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.run(args.photons_per_step)
+        kernel_ms.append(eng.last_kernel_ms())
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    st = eng.stats()
+    total = args.steps * args.photons_per_step
+    out = {"metric": "primary photons/s on 1 MI355X (mesh CT)", "value": total / wall, "unit": "photons/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+           "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "two icosphere meshes (r 50 mm, 81,920 triangles, soft tissue; r 15 mm, bone), "
+                                  "80 keV cone beam at SID 600 mm, curved 888x64 GOS detector at SDD 1040 mm, "
+                                  "dose map on the outer mesh",
+                      "photons_per_step": args.photons_per_step,
+                      "mesh_steps_per_primary": st["mesh_steps"] / total,
+                      "mesh_interactions_per_primary": st["mesh_interactions"] / total},
+           "kernel_ms": sum(kernel_ms) / len(kernel_ms), "kernel_variant": "general"}
+    g.delete()
+    del keep
+    print(json.dumps(out), flush=True)
+
+
+def bench_acquisition(args):
+    """360-projection acquisitions around the three ellipsoid phantoms, each
+    one captured launch; value = all primaries / summed transport time"""
+    from ggems_amd import configs
+    from ggems_amd.ggems import GGEMS
+    lo, hi = int(1_000_000 * args.budget_scale), int(30_000_000 * args.budget_scale)
+    per = []
+    for k, side in enumerate((128, 256, 384)):
+        wd = os.path.join(REPO, "ggems_amd", "data", "_gen", "acquisition")
+        keep = configs.acquisition(wd, side=side, seed=1 + k)
+        rot, budgets = configs.acquisition_plan(n_angles=args.angles, budget_range=(lo, hi), seed=5 + k)
+        g = GGEMS()
+        g.initialize(args.seed + k)
+        g.autotune(4_000_000)
+        g.run_acquisition(rot, budgets, "deg")
+        st = g.engine_stats()
+        ms = g.run_time_ms()
+        n = int(budgets.sum())
+        assert st["primaries"] == n, st
+        per.append({"side": side, "primaries": n, "transport_ms": ms, "photons_per_s": n / ms * 1e3,
+                    "voxel_steps_per_primary": st["voxel_steps"] / n})
+        g.delete()
+        del keep
+    total = sum(p["primaries"] for p in per)
+    t = sum(p["transport_ms"] for p in per) * 1e-3
+    out = {"metric": "primary photons/s on 1 MI355X (CT acquisition, one launch per phantom)", "value": total / t,
+           "unit": "photons/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "%d projections (1 degree) around seeded ellipsoid phantoms of 128^3, 256^3, "
+                                  "384^3 1 mm voxels, budgets uniform in [%d, %d], flat 512x384 GOS panel of "
+                                  "0.78 mm at SDD 1000 mm, 120 kVp" % (args.angles, lo, hi),
+                      "phantoms": per}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,11 +194,20 @@ def main():
     ap.add_argument("--general-kernel", action="store_true", help="run the general kernel variant")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run a plain device copy of this size (calibrates profiler byte counters)")
+    ap.add_argument("--workload", choices=("voxel_ct", "mesh_ct", "acquisition"), default="voxel_ct")
+    ap.add_argument("--angles", type=int, default=360, help="acquisition: projections per phantom")
+    ap.add_argument("--budget-scale", type=float, default=0.1,
+                    help="acquisition: budgets uniform in [1e6, 3e7] times this")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: measures one GPU (--gpus 1); the photon budget of a run shards across GPUs by "
               "particle slots, with one sum-reduction of the tallies at the end", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "mesh_ct":
+        return bench_mesh_ct(args)
+    if args.workload == "acquisition":
+        return bench_acquisition(args)
 
     import torch
 

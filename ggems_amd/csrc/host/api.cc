@@ -8,6 +8,7 @@
 #include <iostream>
 
 #include "simulation.hh"
+#include "volume_creator.hh"
 // checks every definition below against the published declarations
 #include "../../../include/ggems.h"
 
@@ -498,6 +499,104 @@ void attach_to_navigator_dosimetry_calculator(GGEMSDosimetryCalculator* d, char 
     raise("GGEMSDosimetryCalculator", "dosimetry attaches to a voxelized or meshed phantom");
   d->navigator = n;
   n->dosimetry = d;
+  API_CATCH()
+}
+
+// ---- volume creator ------------------------------------------------------------------------------
+GGEMSVolumeCreatorManager* get_instance_volume_creator_manager(void) { return &GGEMSVolumeCreatorManager::get(); }
+void set_volume_dimension_volume_creator_manager(GGEMSVolumeCreatorManager* m, size_t w, size_t h, size_t d) {
+  m->set_dimensions(w, h, d);
+}
+void set_element_sizes_volume_creator_manager(GGEMSVolumeCreatorManager* m, float x, float y, float z, char const* u) {
+  API_TRY m->set_element_sizes(x, y, z, str(u));
+  API_CATCH()
+}
+void set_output_image_filename_volume_creator_manager(GGEMSVolumeCreatorManager* m, char const* f) {
+  m->output_image = str(f);
+}
+void set_output_range_to_material_filename_volume_creator_manager(GGEMSVolumeCreatorManager* m, char const* f) {
+  m->output_range = str(f);
+}
+void initialize_volume_creator_manager(GGEMSVolumeCreatorManager* m) {
+  API_TRY m->initialize();
+  API_CATCH()
+}
+void write_volume_creator_manager(GGEMSVolumeCreatorManager* m) {
+  API_TRY m->write();
+  API_CATCH()
+}
+void set_material_volume_creator_manager(GGEMSVolumeCreatorManager* m, char const* material) {
+  m->set_material(str(material));
+}
+void set_data_type_volume_creator_manager(GGEMSVolumeCreatorManager* m, char const* t) {
+  API_TRY m->set_data_type(str(t));
+  API_CATCH()
+}
+void clean_volume_creator_manager(GGEMSVolumeCreatorManager* m) { m->clean(); }
+GGEMSBox* create_box(float w, float h, float d, char const* u) {
+  API_TRY return new GGEMSBox(w, h, d, str(u));
+  API_CATCH(nullptr)
+}
+GGEMSTube* create_tube(float rx, float ry, float h, char const* u) {
+  API_TRY return new GGEMSTube(rx, ry, h, str(u));
+  API_CATCH(nullptr)
+}
+GGEMSSphere* create_sphere(float r, char const* u) {
+  API_TRY return new GGEMSSphere(r, str(u));
+  API_CATCH(nullptr)
+}
+void delete_box(GGEMSBox* v) { delete v; }
+void set_position_box(GGEMSBox* v, float x, float y, float z, char const* u) {
+  API_TRY v->set_position(x, y, z, str(u));
+  API_CATCH()
+}
+void set_material_box(GGEMSBox* v, char const* material) {
+  API_TRY v->set_material(str(material));
+  API_CATCH()
+}
+void set_label_value_box(GGEMSBox* v, float label) { v->set_label(label); }
+void initialize_box(GGEMSBox* v) {
+  API_TRY v->initialize();
+  API_CATCH()
+}
+void draw_box(GGEMSBox* v) {
+  API_TRY v->draw();
+  API_CATCH()
+}
+void delete_tube(GGEMSTube* v) { delete v; }
+void set_position_tube(GGEMSTube* v, float x, float y, float z, char const* u) {
+  API_TRY v->set_position(x, y, z, str(u));
+  API_CATCH()
+}
+void set_material_tube(GGEMSTube* v, char const* material) {
+  API_TRY v->set_material(str(material));
+  API_CATCH()
+}
+void set_label_value_tube(GGEMSTube* v, float label) { v->set_label(label); }
+void initialize_tube(GGEMSTube* v) {
+  API_TRY v->initialize();
+  API_CATCH()
+}
+void draw_tube(GGEMSTube* v) {
+  API_TRY v->draw();
+  API_CATCH()
+}
+void delete_sphere(GGEMSSphere* v) { delete v; }
+void set_position_sphere(GGEMSSphere* v, float x, float y, float z, char const* u) {
+  API_TRY v->set_position(x, y, z, str(u));
+  API_CATCH()
+}
+void set_material_sphere(GGEMSSphere* v, char const* material) {
+  API_TRY v->set_material(str(material));
+  API_CATCH()
+}
+void set_label_value_sphere(GGEMSSphere* v, float label) { v->set_label(label); }
+void initialize_sphere(GGEMSSphere* v) {
+  API_TRY v->initialize();
+  API_CATCH()
+}
+void draw_sphere(GGEMSSphere* v) {
+  API_TRY v->draw();
   API_CATCH()
 }
 

@@ -290,6 +290,114 @@ class GGEMSVoxelizedPhantom:
         pass
 
 
+class GGEMSVolumeCreatorManager:
+    """Label volume painted on the device, written as MHD/raw plus a range file"""
+
+    def __init__(self):
+        self.obj = _call("get_instance_volume_creator_manager", VP, [])
+
+    def set_dimensions(self, width, height, depth):
+        _call("set_volume_dimension_volume_creator_manager", None, [VP, SZ, SZ, SZ], self.obj, width, height, depth)
+
+    def set_element_sizes(self, width, height, depth, unit):
+        _call("set_element_sizes_volume_creator_manager", None, [VP, F, F, F, S], self.obj, width, height, depth,
+              _b(unit))
+
+    def set_output(self, output):
+        _call("set_output_image_filename_volume_creator_manager", None, [VP, S], self.obj, _b(output))
+
+    def set_range_output(self, output):
+        _call("set_output_range_to_material_filename_volume_creator_manager", None, [VP, S], self.obj, _b(output))
+
+    def initialize(self):
+        _call("initialize_volume_creator_manager", None, [VP], self.obj)
+
+    def write(self):
+        _call("write_volume_creator_manager", None, [VP], self.obj)
+
+    def set_material(self, material):
+        _call("set_material_volume_creator_manager", None, [VP, S], self.obj, _b(material))
+
+    def set_data_type(self, data_type):
+        _call("set_data_type_volume_creator_manager", None, [VP, S], self.obj, _b(data_type))
+
+    def clean(self):
+        _call("clean_volume_creator_manager", None, [VP], self.obj)
+
+
+class GGEMSTube:
+    """GGEMSVolume tube painted into the volume creator's label volume"""
+
+    def __init__(self, radius_x, radius_y, height, unit):
+        self.obj = _call("create_tube", VP, [F, F, F, S], radius_x, radius_y, height, _b(unit))
+
+    def delete(self):
+        _call("delete_tube", None, [VP], self.obj)
+
+    def set_label_value(self, label_value):
+        _call("set_label_value_tube", None, [VP, F], self.obj, label_value)
+
+    def set_position(self, pos_x, pos_y, pos_z, unit):
+        _call("set_position_tube", None, [VP, F, F, F, S], self.obj, pos_x, pos_y, pos_z, _b(unit))
+
+    def set_material(self, material):
+        _call("set_material_tube", None, [VP, S], self.obj, _b(material))
+
+    def initialize(self):
+        _call("initialize_tube", None, [VP], self.obj)
+
+    def draw(self):
+        _call("draw_tube", None, [VP], self.obj)
+
+class GGEMSBox:
+    """GGEMSVolume box painted into the volume creator's label volume"""
+
+    def __init__(self, width, height, depth, unit):
+        self.obj = _call("create_box", VP, [F, F, F, S], width, height, depth, _b(unit))
+
+    def delete(self):
+        _call("delete_box", None, [VP], self.obj)
+
+    def set_label_value(self, label_value):
+        _call("set_label_value_box", None, [VP, F], self.obj, label_value)
+
+    def set_position(self, pos_x, pos_y, pos_z, unit):
+        _call("set_position_box", None, [VP, F, F, F, S], self.obj, pos_x, pos_y, pos_z, _b(unit))
+
+    def set_material(self, material):
+        _call("set_material_box", None, [VP, S], self.obj, _b(material))
+
+    def initialize(self):
+        _call("initialize_box", None, [VP], self.obj)
+
+    def draw(self):
+        _call("draw_box", None, [VP], self.obj)
+
+class GGEMSSphere:
+    """GGEMSVolume sphere painted into the volume creator's label volume"""
+
+    def __init__(self, radius, unit):
+        self.obj = _call("create_sphere", VP, [F, S], radius, _b(unit))
+
+    def delete(self):
+        _call("delete_sphere", None, [VP], self.obj)
+
+    def set_label_value(self, label_value):
+        _call("set_label_value_sphere", None, [VP, F], self.obj, label_value)
+
+    def set_position(self, pos_x, pos_y, pos_z, unit):
+        _call("set_position_sphere", None, [VP, F, F, F, S], self.obj, pos_x, pos_y, pos_z, _b(unit))
+
+    def set_material(self, material):
+        _call("set_material_sphere", None, [VP, S], self.obj, _b(material))
+
+    def initialize(self):
+        _call("initialize_sphere", None, [VP], self.obj)
+
+    def draw(self):
+        _call("draw_sphere", None, [VP], self.obj)
+
+
 class GGEMSMeshedPhantom:
     def __init__(self, meshed_phantom_name):
         self.obj = _call("create_ggems_meshed_phantom", VP, [S], _b(meshed_phantom_name))

@@ -227,6 +227,44 @@ void attach_to_navigator_dosimetry_calculator(GGEMSDosimetryCalculator* dose_cal
 /* ---- additions with no counterpart in GGEMS (tests, benchmarks, tools) ----- */
 /* message of the most recent failed call on this thread not yet read ("" if none);
  * reading it clears it, so call it after each call whose success matters */
+/* ---- volume creator (src/geometries/GGEMSVolumeCreatorManager.cc:262-384,
+ * GGEMSBox.cc:171-231, GGEMSTube.cc:171-231, GGEMSSphere.cc:167-227) ---- */
+GGEMSVolumeCreatorManager* get_instance_volume_creator_manager(void);
+void set_volume_dimension_volume_creator_manager(GGEMSVolumeCreatorManager* volume_creator_manager,
+                                                 size_t volume_width, size_t volume_height, size_t volume_depth);
+void set_element_sizes_volume_creator_manager(GGEMSVolumeCreatorManager* volume_creator_manager, float voxel_width,
+                                              float voxel_height, float voxel_depth, char const* unit);
+void set_output_image_filename_volume_creator_manager(GGEMSVolumeCreatorManager* volume_creator_manager,
+                                                      char const* output_image_filename);
+void set_output_range_to_material_filename_volume_creator_manager(GGEMSVolumeCreatorManager* volume_creator_manager,
+                                                                  char const* output_range_to_material_filename);
+void initialize_volume_creator_manager(GGEMSVolumeCreatorManager* volume_creator_manager);
+void write_volume_creator_manager(GGEMSVolumeCreatorManager* volume_creator_manager);
+void set_material_volume_creator_manager(GGEMSVolumeCreatorManager* volume_creator_manager, char const* material);
+void set_data_type_volume_creator_manager(GGEMSVolumeCreatorManager* volume_creator_manager, char const* data_type);
+void clean_volume_creator_manager(GGEMSVolumeCreatorManager* volume_creator_manager);
+GGEMSBox* create_box(float width, float height, float depth, char const* unit);
+void delete_box(GGEMSBox* box);
+void set_position_box(GGEMSBox* box, float pos_x, float pos_y, float pos_z, char const* unit);
+void set_material_box(GGEMSBox* box, char const* material);
+void set_label_value_box(GGEMSBox* box, float label_value);
+void initialize_box(GGEMSBox* box);
+void draw_box(GGEMSBox* box);
+GGEMSTube* create_tube(float radius_x, float radius_y, float height, char const* unit);
+void delete_tube(GGEMSTube* tube);
+void set_position_tube(GGEMSTube* tube, float pos_x, float pos_y, float pos_z, char const* unit);
+void set_material_tube(GGEMSTube* tube, char const* material);
+void set_label_value_tube(GGEMSTube* tube, float label_value);
+void initialize_tube(GGEMSTube* tube);
+void draw_tube(GGEMSTube* tube);
+GGEMSSphere* create_sphere(float radius, char const* unit);
+void delete_sphere(GGEMSSphere* sphere);
+void set_position_sphere(GGEMSSphere* sphere, float pos_x, float pos_y, float pos_z, char const* unit);
+void set_material_sphere(GGEMSSphere* sphere, char const* material);
+void set_label_value_sphere(GGEMSSphere* sphere, float label_value);
+void initialize_sphere(GGEMSSphere* sphere);
+void draw_sphere(GGEMSSphere* sphere);
+
 /* CT acquisition in one launch (captured as a hipGraph): projection p turns
  * the X-ray source and every CT system by (rx[p], ry[p], rz[p]) [unit] on
  * top of their placement - the matrices SetRotation would give - and

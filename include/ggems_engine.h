@@ -222,6 +222,30 @@ int ggh_get_dose_tallies(ggh_context* ctx, double* edep, double* edep_squared, i
 int ggh_compute_dose(ggh_context* ctx, float scale_factor, int32_t water_reference, float minimum_density,
                      float* dose, float* uncertainty);
 
+/* ---- voxelized volume painter ----------------------------------------------
+ * GGEMSVolumeCreatorManager's label volume (src/geometries/
+ * GGEMSVolumeCreatorManager.cc:170-232) and its box / tube / sphere painters
+ * (src/kernels/DrawGGEMS{Box,Tube,Sphere}.cl), on the device. */
+typedef struct ggh_volume ggh_volume;
+enum { GGH_MET_CHAR = 0, GGH_MET_UCHAR, GGH_MET_SHORT, GGH_MET_USHORT, GGH_MET_INT, GGH_MET_UINT, GGH_MET_FLOAT };
+enum { GGH_SHAPE_BOX = 0, GGH_SHAPE_TUBE = 1, GGH_SHAPE_SPHERE = 2 };
+typedef struct {
+  int32_t shape;
+  float position[3];  /* mm, from the volume centre */
+  float label;        /* converted to the element type */
+  float size[3];      /* mm.  box: height (y), width (x), depth (z); tube: height (z),
+                         radius x, radius y; sphere: radius */
+} ggh_volume_shape;
+const char* ggh_volume_last_error(void);
+/* zero-filled volume of dims[0]*dims[1]*dims[2] elements of data_type (x fastest) */
+int ggh_volume_create(int device, const int32_t dims[3], const float element_size[3], int32_t data_type,
+                      ggh_volume** out);
+/* paints every voxel whose centre lies in the shape (later shapes overwrite earlier ones) */
+int ggh_volume_draw(ggh_volume* volume, const ggh_volume_shape* shape);
+/* copies the volume to host memory (bytes must be its exact size) */
+int ggh_volume_read(ggh_volume* volume, void* host, size_t bytes);
+int ggh_volume_destroy(ggh_volume* volume);
+
 /* ---- test entry points -------------------------------------------------- */
 /* voxel indices visited by rays crossing voxel solid `solid` with no
  * interaction; pos/dir are global.  out_index[r*max_steps + k], out_count[r]. */

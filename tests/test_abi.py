@@ -37,10 +37,6 @@ NOT_EXPORTED = {
     "GGEMS/graphics/GGEMSOpenGLManager.hh": "OpenGL display, out of scope",
     "GGEMS/navigators/GGEMSWorld.hh": "world-grid photon tracking, outside the transport hot path",
     "GGEMS/sources/GGEMSVoxelizedSource.hh": "voxelized (emission) source, outside the transport hot path",
-    "GGEMS/geometries/GGEMSVolumeCreatorManager.hh": "volume creator, not built yet",
-    "GGEMS/geometries/GGEMSBox.hh": "volume creator, not built yet",
-    "GGEMS/geometries/GGEMSTube.hh": "volume creator, not built yet",
-    "GGEMS/geometries/GGEMSSphere.hh": "volume creator, not built yet",
 }
 
 
