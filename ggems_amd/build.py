@@ -24,7 +24,7 @@ ARCH = os.environ.get("GGEMS_OFFLOAD_ARCH", "gfx950")
 HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-ENGINE_SOURCES = [os.path.join(CSRC, "engine", "engine.hip"), os.path.join(CSRC, "engine", "volume.hip")]
+ENGINE_SOURCES = [os.path.join(CSRC, "engine", f) for f in ("engine.hip", "volume.hip", "unet.hip")]
 HOST_SOURCES = sorted(os.path.join(CSRC, "host", f) for f in os.listdir(os.path.join(CSRC, "host"))
                       if f.endswith(".cc")) if os.path.isdir(os.path.join(CSRC, "host")) else []
 

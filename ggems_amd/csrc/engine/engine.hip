@@ -1087,6 +1087,120 @@ __global__ __launch_bounds__(kDoseBlock) void dose_kernel(DoseParams D) {
 }
 
 // ---------------------------------------------------------------------------
+// ray-traced primary transmission (DRR)
+// ---------------------------------------------------------------------------
+// One thread per detector pixel: the segment from the focal point to the
+// pixel centre crosses every voxel solid; the exact length inside each
+// voxel (a 3D DDA in FP64) is added to the voxel's global material in LDS.
+// The transmission is then sum_b w_b exp(-sum_m sigma_tot(m, b) L_m) over
+// the energy bins b the source spectrum reaches, sigma_tot being the
+// left-bin total of the activated processes the transport uses for its
+// distances (GGEMSPhotonNavigator.hh:57-93).
+constexpr int kDrrBlock = 128;
+constexpr int kDrrMaxMat = 32;
+
+struct DrrParams {
+  const DevSolid* solids;
+  int32_t n_solids;
+  int32_t box;
+  Mat34 src_m, box_m;
+  const float4* sig;
+  int32_t nbins, n_mat;
+  const int32_t* w_bin;
+  const float* w;
+  int32_t n_w;
+  float* image;
+};
+
+__device__ void drr_voxel_solid(const DevSolid& s, f3 gs, f3 ge, double* L) {
+  const VoxGeom& G = s.vox;
+  const f3 ls = g2l_pos(s.m, gs), le = g2l_pos(s.m, ge);
+  const double o[3] = {ls.x, ls.y, ls.z};
+  const double d[3] = {(double)le.x - ls.x, (double)le.y - ls.y, (double)le.z - ls.z};
+  const double lo[3] = {s.border.lo.x, s.border.lo.y, s.border.lo.z};
+  const double hi[3] = {s.border.hi.x, s.border.hi.y, s.border.hi.z};
+  const double vs[3] = {G.vs.x, G.vs.y, G.vs.z};
+  const int n[3] = {G.nx, G.ny, G.nz};
+  double t0 = 0.0, t1 = 1.0;
+  for (int a = 0; a < 3; ++a) {
+    if (fabs(d[a]) < 1e-300) {
+      if (o[a] < lo[a] || o[a] > hi[a]) return;
+    } else {
+      const double ta = (lo[a] - o[a]) / d[a], tb = (hi[a] - o[a]) / d[a];
+      t0 = fmax(t0, fmin(ta, tb));
+      t1 = fmin(t1, fmax(ta, tb));
+    }
+  }
+  if (!(t0 < t1)) return;
+  const double len = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  int i[3], step[3];
+  double tmax[3], tdel[3];
+  const double tm = 0.5 * (t0 + fmin(t1, t0 + 1e-9));
+  for (int a = 0; a < 3; ++a) {
+    const double p = o[a] + d[a] * tm;
+    int k = (int)floor((p - lo[a]) / vs[a]);
+    i[a] = k < 0 ? 0 : (k >= n[a] ? n[a] - 1 : k);
+    if (d[a] > 0.0) {
+      step[a] = 1;
+      tmax[a] = (lo[a] + (i[a] + 1) * vs[a] - o[a]) / d[a];
+      tdel[a] = vs[a] / d[a];
+    } else if (d[a] < 0.0) {
+      step[a] = -1;
+      tmax[a] = (lo[a] + i[a] * vs[a] - o[a]) / d[a];
+      tdel[a] = -vs[a] / d[a];
+    } else {
+      step[a] = 0;
+      tmax[a] = INFINITY;
+      tdel[a] = INFINITY;
+    }
+  }
+  double t = t0;
+  const int max_iter = n[0] + n[1] + n[2] + 3;
+  for (int it = 0; it < max_iter; ++it) {
+    const int a = tmax[0] < tmax[1] ? (tmax[0] < tmax[2] ? 0 : 2) : (tmax[1] < tmax[2] ? 1 : 2);
+    const double tn = fmin(tmax[a], t1);
+    if (tn > t) {
+      const int gm = G.mat_offset + G.labels[i[0] + i[1] * G.nx + i[2] * G.nxy];
+      L[gm] += (tn - t) * len;
+    }
+    if (tn >= t1) break;
+    t = tn;
+    i[a] += step[a];
+    if (i[a] < 0 || i[a] >= n[a]) break;
+    tmax[a] += tdel[a];
+  }
+}
+
+__global__ __launch_bounds__(kDrrBlock) void drr_kernel(DrrParams D) {
+  __shared__ double sL[kDrrBlock * kDrrMaxMat];
+  double* L = sL + threadIdx.x * kDrrMaxMat;
+  for (int m = 0; m < kDrrMaxMat; ++m) L[m] = 0.0;
+  const DevSolid& box = D.solids[D.box];
+  const int npx = box.ne[0] * box.ne[1];
+  const int px = blockIdx.x * kDrrBlock + threadIdx.x;
+  if (px >= npx) return;
+  const int ix = px % box.ne[0], iy = px / box.ne[0];
+  const float esx = box.box_size.x / (float)box.ne[0], esy = box.box_size.y / (float)box.ne[1];
+  const f3 pc = mk(box.border.lo.x + ((float)ix + 0.5f) * esx, box.border.lo.y + ((float)iy + 0.5f) * esy,
+                   0.5f * (box.border.lo.z + box.border.hi.z));
+  const f3 ge = l2g_pos(D.box_m, pc);
+  const f3 gs = l2g_pos(D.src_m, mk(0.0f, 0.0f, 0.0f));
+  for (int k = 0; k < D.n_solids; ++k)
+    if (D.solids[k].type == kVoxel) drr_voxel_solid(D.solids[k], gs, ge, L);
+  double t = 0.0;
+  for (int j = 0; j < D.n_w; ++j) {
+    const int b = D.w_bin[j];
+    double mu = 0.0;
+    for (int m = 0; m < D.n_mat; ++m) {
+      const float4 sg = D.sig[m * D.nbins + b];
+      mu += ((double)sg.x + (double)sg.y + (double)sg.z) * L[m];
+    }
+    t += (double)D.w[j] * exp(-mu);
+  }
+  D.image[px] = (float)t;
+}
+
+// ---------------------------------------------------------------------------
 // test kernels
 // ---------------------------------------------------------------------------
 __global__ void walk_kernel(RunParams P, int solid, int n_rays, const float* pos, const float* dir, int max_steps,
@@ -1224,6 +1338,8 @@ struct ggh_context {
   int nbins = 0, n_mat = 0, n_proc = 0, proc[3] = {0, 0, 0};
   float factor = 0.0f;
   std::vector<float> density_host;
+  std::vector<float> ebins_host;      // energy grid and step tables, for the ray tracer's weights
+  std::vector<float4> sig_host;
   DevBuf<float> ebins, cs_r, cs_atom_r, atom_dens, pp, density;
   DevBuf<float4> tab_sig, tab_rcp;
   DevBuf<int32_t> n_elem, elem_idx;
@@ -1247,6 +1363,7 @@ struct ggh_context {
   f3 focal{0, 0, 0};
   int n_energy_arg = 1, is_interp = 0;
   DevBuf<float> spectrum, cdf;
+  std::vector<float> spectrum_host, cdf_host;
   // slots
   DevBuf<uint32_t> rng;
   DevBuf<uint32_t> queue;
@@ -1265,6 +1382,7 @@ struct ggh_context {
   DevBuf<Mat34> acq_src, acq_box;
   DevBuf<uint4> acq_layout;
   std::vector<DevBuf<int32_t>> acq_hist, acq_scat;  // per solid: n_proj images (box solids)
+  std::vector<Mat34> acq_src_host, acq_box_host;
   DevBuf<DevSolid> acq_solids;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -1399,6 +1517,8 @@ void drop_acquisition(ggh_context* c) {
   c->acq_scat.clear();
   c->acq_n_proj = 0;
   c->acq_slots = 0;
+  c->acq_src_host.clear();
+  c->acq_box_host.clear();
 }
 
 // bytes of the LDS image the transport kernel stages per block: step tables,
@@ -1609,6 +1729,8 @@ int ggh_set_physics(ggh_context* c, const ggh_physics_desc* d) {
   for (int i = 0; i < 3; ++i) c->proc[i] = i < d->n_processes ? d->process_id[i] : 0;
   c->factor = d->rayleigh_factor;
   c->density_host.assign(d->density, d->density + nm);
+  c->ebins_host.assign(d->energy_bins, d->energy_bins + nb);
+  c->sig_host = sig;
   if ((rc = dev_upload(c->ebins, d->energy_bins, (size_t)nb, c->stream))) return rc;
   if ((rc = dev_upload(c->tab_sig, sig.data(), sig.size(), c->stream))) return rc;
   if ((rc = dev_upload(c->tab_rcp, rcp.data(), rcp.size(), c->stream))) return rc;
@@ -1881,6 +2003,8 @@ int ggh_set_xray_source(ggh_context* c, const ggh_xray_source_desc* d) {
   f.push_back(1.0f);
   if ((rc = dev_upload(c->spectrum, e.data(), e.size(), c->stream))) return rc;
   if ((rc = dev_upload(c->cdf, f.data(), f.size(), c->stream))) return rc;
+  c->spectrum_host = e;
+  c->cdf_host = f;
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->have_source = true;
   return 0;
@@ -1983,6 +2107,8 @@ int ggh_set_acquisition(ggh_context* c, const ggh_acquisition_desc* d) {
   for (int p = 0; p < np; ++p) src[p] = to_mat34(d->source_matrices + 16 * (size_t)p);
   for (size_t k = 0; k < box.size(); ++k) box[k] = to_mat34(d->box_matrices + 16 * k);
   drop_acquisition(c);
+  c->acq_src_host = src;
+  c->acq_box_host = box;
   if ((rc = dev_upload(c->acq_src, src.data(), src.size(), c->stream)) ||
       (rc = dev_upload(c->acq_box, box.data(), box.size(), c->stream)) ||
       (rc = dev_upload(c->acq_layout, lay.data(), lay.size(), c->stream)))
@@ -2058,6 +2184,114 @@ int ggh_replay_acquisition(ggh_context* c) {
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
   HIP_TRY(hipEventSynchronize(c->ev1));
   HIP_TRY(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// BinarySearch over the host energy grid, as the device does it
+int host_bin(const std::vector<float>& a, float key) {
+  int low = 0, high = (int)a.size() - 1;
+  while (high - low > 1) {
+    const int mid = (high + low) / 2;
+    if (a[mid] == key) return mid;
+    if (a[mid] < key) low = mid;
+    else high = mid;
+  }
+  return low;
+}
+
+// Probability of each energy bin of the cross-section tables under the
+// source's sampling (GetPrimariesGGEMSXRaySource.cl:103-125): line idx has
+// probability cdf[idx+1] - cdf[idx]; an interpolated spectrum spreads it
+// uniformly over [E_idx, E_idx+1), split at the bin edges.
+void drr_weights(const ggh_context* c, std::vector<int32_t>& bins, std::vector<float>& w) {
+  const std::vector<float>& eb = c->ebins_host;
+  const int nb = (int)eb.size();
+  std::vector<double> W((size_t)nb, 0.0);
+  const std::vector<float>& e = c->spectrum_host;
+  const std::vector<float>& f = c->cdf_host;
+  for (int idx = 0; idx + 1 < (int)f.size() && idx + 1 < (int)e.size(); ++idx) {
+    const double p = (double)f[idx + 1] - (double)f[idx];
+    if (!(p > 0.0)) continue;
+    const double a = e[idx], b = e[idx + 1];
+    if (!c->is_interp || !(b > a)) {
+      W[host_bin(eb, e[idx])] += p;
+      continue;
+    }
+    for (int k = 0; k + 1 < nb; ++k) {
+      const double lo = k == 0 ? -INFINITY : (double)eb[k];
+      const double hi = k + 2 == nb ? INFINITY : (double)eb[k + 1];
+      const double ov = std::min(b, hi) - std::max(a, lo);
+      if (ov > 0.0) W[k] += p * ov / (b - a);
+    }
+  }
+  bins.clear();
+  w.clear();
+  for (int k = 0; k < nb; ++k)
+    if (W[k] > 0.0) {
+      bins.push_back(k);
+      w.push_back((float)W[k]);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ggh_drr(ggh_context* c, int solid, int projection, float* image) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!image) return fail(-1, "null image");
+  if (!c->have_physics || !c->have_source) return fail(-3, "physics tables and source must be set");
+  if (solid < 0 || solid >= (int)c->solids.size() || c->solids[solid].type != kBoxSolid)
+    return fail(-2, "not a box solid");
+  if (c->n_mat > kDrrMaxMat) return fail(-2, "the ray tracer handles at most 32 materials");
+  for (size_t i = 0; i < c->solids.size(); ++i)
+    if (c->solids[i].type == kVoxel && c->max_material[i] >= c->n_mat)
+      return fail(-3, "solid labels reach a material without physics tables");
+  if ((rc = sync_solids(c))) return rc;
+  DrrParams D;
+  std::memset(&D, 0, sizeof D);
+  const DevSolid& box = c->solids[solid];
+  if (projection < 0) {
+    D.src_m = c->src_m;
+    D.box_m = box.m;
+  } else {
+    const int n_box = n_solids_of(c, kBoxSolid);
+    if (projection >= c->acq_n_proj) return fail(-2, "no such projection");
+    D.src_m = c->acq_src_host[projection];
+    D.box_m = c->acq_box_host[(size_t)projection * n_box + box.box_index];
+  }
+  std::vector<int32_t> bins;
+  std::vector<float> w;
+  drr_weights(c, bins, w);
+  const size_t npx = (size_t)box.ne[0] * box.ne[1];
+  DevBuf<int32_t> db;
+  DevBuf<float> dw, dimg;
+  if ((rc = dev_upload(db, bins.data(), bins.size(), c->stream)) || (rc = dev_upload(dw, w.data(), w.size(), c->stream)) ||
+      (rc = dev_alloc(dimg, npx)))
+    return rc;
+  D.solids = c->solids_dev.p;
+  D.n_solids = (int32_t)c->solids.size();
+  D.box = solid;
+  D.sig = c->tab_sig.p;
+  D.nbins = c->nbins;
+  D.n_mat = c->n_mat;
+  D.w_bin = db.p;
+  D.w = dw.p;
+  D.n_w = (int32_t)bins.size();
+  D.image = dimg.p;
+  hipLaunchKernelGGL(drr_kernel, dim3((unsigned)((npx + kDrrBlock - 1) / kDrrBlock)), dim3(kDrrBlock), 0, c->stream, D);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(image, dimg.p, npx * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  db.release();
+  dw.release();
+  dimg.release();
+  if (e != hipSuccess) return fail(-100 - (int)e, std::string("ray tracer: ") + hipGetErrorString(e));
   return 0;
 }
 

@@ -1,0 +1,224 @@
+// 3x3 convolutions of the scatter surrogate's U-Net on the matrix cores:
+// BF16 operands, FP32 accumulation (v_mfma_f32_16x16x32_bf16).
+//
+// Layouts: activations NHWC (channels fastest), weights [Cout][9][Cin]
+// (tap = 3 * (dy + 1) + (dx + 1)), so that the GEMM's reduction index
+// k = tap * Cin + ci runs over contiguous memory in both operands.  Cin and
+// Cout are multiples of 8 and 16 (the first layer pads its two input
+// channels); W is a multiple of 16 and N H W of 32.
+//
+// Forward (implicit GEMM, M = pixels, N = Cout, K = 9 Cin): a wave owns
+// 32 consecutive pixels (two runs of 16 within image rows) and 64 output
+// channels, i.e.
+// 2 x 4 accumulator tiles of 16 x 16; each K step of 32 loads one 16-byte
+// A fragment per pixel tile and one B fragment per channel tile and issues
+// 8 MFMAs.  Bias and ReLU are applied in the epilogue.  The data gradient
+// is the same kernel on the flipped, transposed weights.
+//
+// Weight gradient (M = Cout, N = 9 Cin, K = pixels): both operands have
+// pixels as their strided index, so a block stages 32 pixels of dY and of
+// the shifted input in LDS transposed (pixels contiguous) and the waves
+// read MFMA fragments from there; blocks split the pixels and add their
+// partial sums atomically in FP32.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../../include/ggems_engine.h"
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+thread_local std::string g_unet_error;
+
+int ufail(int code, const std::string& msg) {
+  g_unet_error = msg;
+  return code;
+}
+
+constexpr int kWaves = 4;
+constexpr int kFwdPix = 32;  // pixels per wave
+constexpr int kFwdCh = 64;   // output channels per wave
+
+__device__ inline bf16x8 zero8() {
+  bf16x8 z;
+  for (int i = 0; i < 8; ++i) z[i] = (__bf16)0.0f;
+  return z;
+}
+
+// A fragment: 8 consecutive k of pixel (y, x) for reduction index k0
+// (k0 multiple of 8; all 8 lie in one tap because Cin is a multiple of 8)
+__device__ inline bf16x8 load_a(const __bf16* x, int img, int y, int xx, int k0, int K, int Cin, int H, int W) {
+  if (k0 >= K) return zero8();
+  const int tap = k0 / Cin, ci = k0 - tap * Cin;
+  const int yy = y + tap / 3 - 1, xs = xx + tap % 3 - 1;
+  if (yy < 0 || yy >= H || xs < 0 || xs >= W) return zero8();
+  return *reinterpret_cast<const bf16x8*>(x + (((size_t)img * H + yy) * W + xs) * Cin + ci);
+}
+
+__global__ __launch_bounds__(64 * kWaves) void conv_fwd_kernel(const __bf16* __restrict__ x,
+                                                               const __bf16* __restrict__ w,
+                                                               const float* __restrict__ bias,
+                                                               __bf16* __restrict__ y, int N, int H, int W, int Cin,
+                                                               int Cout, int relu) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int pix_tiles = (N * H * W) / kFwdPix;
+  const int ch_tiles = (Cout + kFwdCh - 1) / kFwdCh;
+  const int job = blockIdx.x * kWaves + wave;
+  if (job >= pix_tiles * ch_tiles) return;
+  const int pt = job / ch_tiles, ct = job - pt * ch_tiles;
+  const int p0 = pt * kFwdPix;
+  // each run of 16 pixels lies in one image row (W is a multiple of 16)
+  int img[2], yrow[2], x0[2];
+  for (int i = 0; i < 2; ++i) {
+    const int q = p0 + 16 * i, rem = q % (H * W);
+    img[i] = q / (H * W);
+    yrow[i] = rem / W;
+    x0[i] = rem - yrow[i] * W;
+  }
+  const int c0 = ct * kFwdCh;
+  const int K = 9 * Cin;
+  const int r = lane & 15, g = lane >> 4;  // fragment row / k group
+  f32x4 acc[2][4];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    const int kk = k0 + 8 * g;
+    bf16x8 a[2], b[4];
+    for (int i = 0; i < 2; ++i) a[i] = load_a(x, img[i], yrow[i], x0[i] + r, kk, K, Cin, H, W);
+    for (int j = 0; j < 4; ++j) {
+      const int co = c0 + 16 * j + r;
+      b[j] = (co < Cout && kk < K) ? *reinterpret_cast<const bf16x8*>(w + (size_t)co * K + kk) : zero8();
+    }
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+  }
+  // C fragment: column r (output channel), rows 4 g .. 4 g + 3 (pixels)
+  for (int j = 0; j < 4; ++j) {
+    const int co = c0 + 16 * j + r;
+    if (co >= Cout) continue;
+    const float bv = bias ? bias[co] : 0.0f;
+    for (int i = 0; i < 2; ++i)
+      for (int e = 0; e < 4; ++e) {
+        float v = acc[i][j][e] + bv;
+        if (relu) v = v > 0.0f ? v : 0.0f;
+        const int px = p0 + 16 * i + 4 * g + e;
+        y[(size_t)px * Cout + co] = (__bf16)v;
+      }
+  }
+}
+
+constexpr int kWgPix = 32;   // pixels per K step
+constexpr int kWgN = 64;     // (tap, ci) columns per block
+constexpr int kWgMaxM = 128; // output channels per block (all of them)
+
+__global__ __launch_bounds__(64 * kWaves) void conv_wgrad_kernel(const __bf16* __restrict__ x,
+                                                                 const __bf16* __restrict__ dy,
+                                                                 float* __restrict__ dw, int N, int H, int W,
+                                                                 int Cin, int Cout, int pix_per_split) {
+  __shared__ __bf16 s_dy[kWgMaxM][kWgPix + 8];  // [co][pixel], padded rows
+  __shared__ __bf16 s_x[kWgN][kWgPix + 8];      // [n][pixel]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int K9 = 9 * Cin;
+  const int n0 = blockIdx.x * kWgN;
+  const int P = N * H * W;
+  const int pbeg = blockIdx.y * pix_per_split;
+  const int pend = min(P, pbeg + pix_per_split);
+  const int r = lane & 15, g = lane >> 4;
+  const int m_tiles = Cout / 16;
+  // wave w owns column tile w (16 of the block's 64 columns) and every row tile
+  f32x4 acc[kWgMaxM / 16];
+  for (int i = 0; i < kWgMaxM / 16; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int pc = pbeg; pc < pend; pc += kWgPix) {
+    // stage dY (32 pixels x Cout) transposed: thread t loads 8 channels of one pixel
+    for (int t = threadIdx.x; t < kWgPix * (Cout / 8); t += 64 * kWaves) {
+      const int p = t / (Cout / 8), c8 = (t - p * (Cout / 8)) * 8;
+      const int px = pc + p;
+      bf16x8 v = px < pend ? *reinterpret_cast<const bf16x8*>(dy + (size_t)px * Cout + c8) : zero8();
+      for (int e = 0; e < 8; ++e) s_dy[c8 + e][p] = v[e];
+    }
+    // stage the shifted input for this block's 64 columns (8 consecutive ci share a tap)
+    for (int t = threadIdx.x; t < kWgPix * (kWgN / 8); t += 64 * kWaves) {
+      const int p = t / (kWgN / 8), n8 = (t - p * (kWgN / 8)) * 8;
+      const int n = n0 + n8, px = pc + p;
+      bf16x8 v = zero8();
+      if (px < pend && n < K9) {
+        const int tap = n / Cin, ci = n - tap * Cin;
+        const int img = px / (H * W), rem = px - img * H * W, yy = rem / W + tap / 3 - 1, xs = rem % W + tap % 3 - 1;
+        if (yy >= 0 && yy < H && xs >= 0 && xs < W)
+          v = *reinterpret_cast<const bf16x8*>(x + (((size_t)img * H + yy) * W + xs) * Cin + ci);
+      }
+      for (int e = 0; e < 8; ++e) s_x[n8 + e][p] = v[e];
+    }
+    __syncthreads();
+    // B fragment: column (n) r of the wave's tile, pixels 8 g .. 8 g + 7
+    bf16x8 b;
+    for (int e = 0; e < 8; ++e) b[e] = s_x[16 * wave + r][8 * g + e];
+#pragma unroll
+    for (int i = 0; i < kWgMaxM / 16; ++i) {
+      if (i >= m_tiles) break;
+      bf16x8 a;
+      for (int e = 0; e < 8; ++e) a[e] = s_dy[16 * i + r][8 * g + e];
+      acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // C fragment: column n = n0 + 16 wave + r, rows co = 16 i + 4 g + e
+  const int n = n0 + 16 * wave + r;
+  if (n >= K9) return;
+#pragma unroll
+  for (int i = 0; i < kWgMaxM / 16; ++i) {
+    if (i >= m_tiles) break;
+    for (int e = 0; e < 4; ++e) {
+      const int co = 16 * i + 4 * g + e;
+      unsafeAtomicAdd(&dw[(size_t)co * K9 + n], acc[i][e]);
+    }
+  }
+}
+
+bool shapes_ok(int N, int H, int W, int Cin, int Cout) {
+  return N > 0 && H > 0 && W > 0 && W % 16 == 0 && ((long long)N * H * W) % kFwdPix == 0 && Cin > 0 &&
+         Cin % 8 == 0 && Cout > 0 && Cout % 16 == 0 && (long long)N * H * W * (Cin > Cout ? Cin : Cout) < (1ll << 31);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ggh_unet_last_error(void) { return g_unet_error.c_str(); }
+
+int ggh_conv3x3_forward(const void* x, const void* w, const float* bias, void* y, int32_t n, int32_t h, int32_t wd,
+                        int32_t cin, int32_t cout, int32_t relu, void* stream) {
+  if (!x || !w || !y) return ufail(-1, "null pointer");
+  if (!shapes_ok(n, h, wd, cin, cout)) return ufail(-2, "unsupported shape (W % 16, N H W % 32, Cin % 8, Cout % 16)");
+  const int jobs = (n * h * wd / kFwdPix) * ((cout + kFwdCh - 1) / kFwdCh);
+  hipLaunchKernelGGL(conv_fwd_kernel, dim3((jobs + kWaves - 1) / kWaves), dim3(64 * kWaves), 0,
+                     static_cast<hipStream_t>(stream), static_cast<const __bf16*>(x), static_cast<const __bf16*>(w),
+                     bias, static_cast<__bf16*>(y), n, h, wd, cin, cout, relu);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return ufail(-100 - (int)e, std::string("conv forward: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int ggh_conv3x3_weight_grad(const void* x, const void* dy, float* dw, int32_t n, int32_t h, int32_t wd, int32_t cin,
+                            int32_t cout, int32_t splits, void* stream) {
+  if (!x || !dy || !dw) return ufail(-1, "null pointer");
+  if (!shapes_ok(n, h, wd, cin, cout) || cout > kWgMaxM) return ufail(-2, "unsupported shape (Cout <= 128)");
+  const int P = n * h * wd;
+  if (splits < 1) splits = 1;
+  int per = (P + splits - 1) / splits;
+  per = (per + kWgPix - 1) / kWgPix * kWgPix;
+  splits = (P + per - 1) / per;
+  const int cols = (9 * cin + kWgN - 1) / kWgN;
+  hipLaunchKernelGGL(conv_wgrad_kernel, dim3(cols, splits), dim3(64 * kWaves), 0, static_cast<hipStream_t>(stream),
+                     static_cast<const __bf16*>(x), static_cast<const __bf16*>(dy), dw, n, h, wd, cin, cout, per);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return ufail(-100 - (int)e, std::string("conv weight gradient: ") + hipGetErrorString(e));
+  return 0;
+}
+
+}  // extern "C"

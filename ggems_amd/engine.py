@@ -128,6 +128,7 @@ def library():
             "ggh_capture_acquisition": ([vp], C.c_int),
             "ggh_replay_acquisition": ([vp], C.c_int),
             "ggh_autotune": ([vp, C.c_uint64, C.POINTER(LaunchDesc)], C.c_int),
+            "ggh_drr": ([vp, C.c_int, C.c_int, vp], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -436,6 +437,14 @@ class Engine:
         scatter = np.zeros((ny, nx), np.int32) if has_scatter else None
         _check(library().ggh_get_projection(self.h, solid, p, _ptr(counts), _ptr(scatter)))
         return counts, scatter
+
+    def drr(self, solid, projection=-1):
+        """ray-traced primary transmission [rows][cols] onto box solid `solid`
+        (projection < 0: the set-up placement; else that acquisition projection)"""
+        ny, nx, _ = self.box_elements[solid]
+        img = np.zeros((ny, nx), np.float32)
+        _check(library().ggh_drr(self.h, solid, int(projection), _ptr(img)))
+        return img
 
     def autotune(self, n_trial):
         """fastest launch shape over a trial run; call after seed() and before any run"""

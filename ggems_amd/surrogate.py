@@ -1,0 +1,202 @@
+"""Learned scatter surrogate: a U-Net that maps a low-count Monte Carlo
+projection and the ray-traced primary image to the scatter image.
+
+Its 3x3 convolutions run on the matrix cores through the engine library
+(ggh_conv3x3_forward / ggh_conv3x3_weight_grad: BF16 operands, FP32
+accumulation); PyTorch holds the tensors, streams and optimiser state.
+Activations are NHWC BF16; master weights are FP32 [cout][9][cin] and are
+cast to BF16 for every call.
+
+Architecture (4 levels, 16-32-64-128 channels): two 3x3 conv + ReLU per
+level, 2x2 max-pooling down, nearest-neighbour 2x upsampling, concatenation
+with the skip, two 3x3 conv + ReLU per decoder level, and a 1x1 head.
+
+The training pairs come from the engine itself: for a seeded phantom and
+gantry angle, a projection of n_low primaries (counts over the open-field
+counts) and the ray-traced transmission are the inputs; the scatter image
+of n_high primaries, per primary and per unit open-field count, is the
+target.  The scatter image is the reference's scatter histogram
+(src/kernels/TrackThroughGGEMSSolidBox.cl:215-225).
+"""
+
+import ctypes as C
+import math
+
+import numpy as np
+import torch
+
+from .engine import EngineError, library
+
+_SIGS = {
+    "ggh_unet_last_error": ([], C.c_char_p),
+    "ggh_conv3x3_forward": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p] + [C.c_int32] * 6 + [C.c_void_p],
+                            C.c_int),
+    "ggh_conv3x3_weight_grad": ([C.c_void_p, C.c_void_p, C.c_void_p] + [C.c_int32] * 6 + [C.c_void_p], C.c_int),
+}
+
+
+def _lib():
+    L = library()
+    if not getattr(L, "_unet_ready", False):
+        for name, (args, res) in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        L._unet_ready = True
+    return L
+
+
+def _check(rc):
+    if rc < 0:
+        raise EngineError(_lib().ggh_unet_last_error().decode(errors="replace") + " (code %d)" % rc)
+
+
+def _stream(t):
+    return C.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
+
+
+def conv3x3(x, w, bias=None, relu=False):
+    """x [N,H,W,Cin] BF16, w [Cout, 9*Cin] BF16, bias [Cout] FP32 -> [N,H,W,Cout] BF16"""
+    if x.device.type != "cuda" or x.dtype != torch.bfloat16 or w.dtype != torch.bfloat16:
+        raise EngineError("conv3x3 takes BF16 tensors on the GPU")
+    x, w = x.contiguous(), w.contiguous()
+    n, h, wd, cin = x.shape
+    cout = w.shape[0]
+    if w.shape[1] != 9 * cin:
+        raise EngineError("weights must be [cout, 9*cin]")
+    y = torch.empty((n, h, wd, cout), dtype=torch.bfloat16, device=x.device)
+    b = None if bias is None else bias.contiguous().float()
+    _check(_lib().ggh_conv3x3_forward(x.data_ptr(), w.data_ptr(), None if b is None else b.data_ptr(), y.data_ptr(),
+                                      n, h, wd, cin, cout, int(bool(relu)), _stream(x)))
+    return y
+
+
+def conv3x3_weight_grad(x, dy, splits=None):
+    """sum over pixels of dy (x) the shifted x: [Cout, 9*Cin] FP32"""
+    x, dy = x.contiguous(), dy.contiguous()
+    n, h, wd, cin = x.shape
+    cout = dy.shape[3]
+    dw = torch.zeros((cout, 9 * cin), dtype=torch.float32, device=x.device)
+    if splits is None:
+        splits = max(1, min(1024, (n * h * wd) // 1024))
+    _check(_lib().ggh_conv3x3_weight_grad(x.data_ptr(), dy.data_ptr(), dw.data_ptr(), n, h, wd, cin, cout,
+                                          int(splits), _stream(x)))
+    return dw
+
+
+def flip_transpose(w):
+    """weights of the data gradient: [cout][9][cin] -> [cin][9][cout], taps reversed"""
+    cout, _, cin = w.shape
+    return w.flip(1).permute(2, 1, 0).contiguous()
+
+
+class _Conv3x3(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, w, b, relu):
+        cout, _, cin = w.shape
+        wb = w.to(torch.bfloat16).contiguous()
+        y = conv3x3(x, wb.view(cout, 9 * cin), b, relu)
+        ctx.save_for_backward(x, wb, y)
+        ctx.relu = relu
+        return y
+
+    @staticmethod
+    def backward(ctx, dy):
+        x, wb, y = ctx.saved_tensors
+        cout, _, cin = wb.shape
+        dy = dy.to(torch.bfloat16).contiguous()
+        if ctx.relu:
+            dy = dy * (y > 0).to(torch.bfloat16)
+        dx = None
+        if ctx.needs_input_grad[0]:
+            dx = conv3x3(dy, flip_transpose(wb).view(cin, 9 * cout))
+        dw = conv3x3_weight_grad(x, dy).view(cout, 9, cin)
+        db = dy.float().sum(dim=(0, 1, 2))
+        return dx, dw, db, None
+
+
+class Conv3x3(torch.nn.Module):
+    def __init__(self, cin, cout, relu=True, generator=None):
+        super().__init__()
+        std = math.sqrt(2.0 / (9 * cin))
+        self.weight = torch.nn.Parameter(torch.randn(cout, 9, cin, generator=generator) * std)
+        self.bias = torch.nn.Parameter(torch.zeros(cout))
+        self.relu = relu
+
+    def forward(self, x):
+        return _Conv3x3.apply(x, self.weight, self.bias, self.relu)
+
+
+def _pool(x):
+    n, h, w, c = x.shape
+    return x.view(n, h // 2, 2, w // 2, 2, c).amax(dim=(2, 4))
+
+
+def _up(x):
+    n, h, w, c = x.shape
+    return x[:, :, None, :, None, :].expand(n, h, 2, w, 2, c).reshape(n, 2 * h, 2 * w, c)
+
+
+IN_CHANNELS = 8  # the two inputs, zero-padded to the kernels' channel multiple
+
+
+class UNet(torch.nn.Module):
+    """4-level U-Net, 16-32-64-128 channels, on NHWC BF16 tensors"""
+
+    def __init__(self, channels=(16, 32, 64, 128), seed=0):
+        super().__init__()
+        g = torch.Generator().manual_seed(seed)
+        c = channels
+        self.enc = torch.nn.ModuleList()
+        cin = IN_CHANNELS
+        for ch in c:
+            self.enc.append(torch.nn.ModuleList([Conv3x3(cin, ch, generator=g), Conv3x3(ch, ch, generator=g)]))
+            cin = ch
+        self.dec = torch.nn.ModuleList()
+        for skip in reversed(c[:-1]):
+            self.dec.append(torch.nn.ModuleList([Conv3x3(cin + skip, skip, generator=g), Conv3x3(skip, skip,
+                                                                                                   generator=g)]))
+            cin = skip
+        self.head = torch.nn.Parameter(torch.randn(c[0], generator=g) * math.sqrt(1.0 / c[0]))
+        self.head_bias = torch.nn.Parameter(torch.zeros(1))
+
+    def forward(self, x):
+        """x [N, H, W, 2] (any float type) -> [N, H, W] FP32; H, W multiples of 256 / 8"""
+        n, h, w, k = x.shape
+        z = torch.zeros((n, h, w, IN_CHANNELS), dtype=torch.bfloat16, device=x.device)
+        z[..., :k] = x.to(torch.bfloat16)
+        skips = []
+        for i, (a, b) in enumerate(self.enc):
+            z = b(a(z))
+            if i + 1 < len(self.enc):
+                skips.append(z)
+                z = _pool(z)
+        for (a, b), s in zip(self.dec, reversed(skips)):
+            z = b(a(torch.cat([_up(z), s], dim=3)))
+        return z.float() @ self.head + self.head_bias
+
+
+def reference_forward(net, x):
+    """The same network with torch's own convolutions (NCHW) in the dtype
+    and on the device of its parameters (FP32 or FP64 on the CPU for the
+    gradient checks): x [N, H, W, 2] -> [N, H, W]."""
+    import torch.nn.functional as Fn
+
+    def conv(mod, z):
+        cout, _, cin = mod.weight.shape
+        w = mod.weight.view(cout, 3, 3, cin).permute(0, 3, 1, 2)
+        y = Fn.conv2d(z, w, mod.bias, padding=1)
+        return torch.relu(y) if mod.relu else y
+
+    n, h, w, k = x.shape
+    z = torch.zeros((n, IN_CHANNELS, h, w), dtype=net.head.dtype, device=net.head.device)
+    z[:, :k] = x.permute(0, 3, 1, 2).to(net.head.dtype)
+    skips = []
+    for i, (a, b) in enumerate(net.enc):
+        z = conv(b, conv(a, z))
+        if i + 1 < len(net.enc):
+            skips.append(z)
+            z = Fn.max_pool2d(z, 2)
+    for (a, b), s in zip(net.dec, reversed(skips)):
+        z = conv(b, conv(a, torch.cat([Fn.interpolate(z, scale_factor=2, mode="nearest"), s], dim=1)))
+    return torch.einsum("nchw,c->nhw", z, net.head) + net.head_bias

@@ -212,6 +212,18 @@ int ggh_get_projection(ggh_context* ctx, int solid, int projection, int32_t* cou
 int ggh_capture_acquisition(ggh_context* ctx);
 int ggh_replay_acquisition(ggh_context* ctx);
 
+/* Ray-traced primary transmission (DRR) onto the pixels of box solid
+ * `solid`: image[row][col] = sum_b w_b exp(-sum_m sigma_tot(m, b) L_m), the
+ * probability that a source photon aimed at the pixel centre crosses the
+ * voxel solids without interacting.  L_m is the exact path length in
+ * material m, sigma_tot the left-bin total cross section the transport uses
+ * for its distances (include/GGEMS/navigators/GGEMSPhotonNavigator.hh:57-93),
+ * w_b the probability of energy bin b under the source's sampling
+ * (src/kernels/GetPrimariesGGEMSXRaySource.cl:103-125).  projection < 0:
+ * the set-up placement; otherwise that projection of the set acquisition.
+ * No counterpart in GGEMS: it feeds the scatter surrogate. */
+int ggh_drr(ggh_context* ctx, int solid, int projection, float* image);
+
 /* Launch autotuner: times ggh_run(n_trial) over a grid of launch shapes
  * (blocks per compute unit, hot threshold, interaction batch) and keeps the
  * fastest.  Call after ggh_seed and before any run: it restores the seeded
@@ -245,6 +257,21 @@ int ggh_volume_draw(ggh_volume* volume, const ggh_volume_shape* shape);
 /* copies the volume to host memory (bytes must be its exact size) */
 int ggh_volume_read(ggh_volume* volume, void* host, size_t bytes);
 int ggh_volume_destroy(ggh_volume* volume);
+
+/* ---- scatter-surrogate U-Net: 3x3 convolutions on the matrix cores ---------
+ * No counterpart in GGEMS (the surrogate is new).  BF16 NHWC activations,
+ * BF16 weights [cout][9][cin] (tap = 3 (dy + 1) + (dx + 1)), FP32
+ * accumulation; zero padding 1.  Device pointers; `stream` is a hipStream_t
+ * (null: the default stream).  W % 16 == 0, N H W % 32 == 0, cin % 8 == 0,
+ * cout % 16 == 0. */
+const char* ggh_unet_last_error(void);
+/* y = act(conv(x, w) + bias); act = ReLU when relu != 0; bias may be null */
+int ggh_conv3x3_forward(const void* x, const void* w, const float* bias, void* y, int32_t n, int32_t h, int32_t w_px,
+                        int32_t cin, int32_t cout, int32_t relu, void* stream);
+/* dw[cout][9 cin] += sum over pixels of dy (x) shifted x, FP32 (cout <= 128);
+ * the pixels are split over `splits` groups of blocks that add atomically */
+int ggh_conv3x3_weight_grad(const void* x, const void* dy, float* dw, int32_t n, int32_t h, int32_t w_px, int32_t cin,
+                            int32_t cout, int32_t splits, void* stream);
 
 /* ---- test entry points -------------------------------------------------- */
 /* voxel indices visited by rays crossing voxel solid `solid` with no

@@ -1,0 +1,91 @@
+"""The surrogate's MFMA convolutions against torch's FP32 convolutions on
+the CPU (same BF16-rounded operands), the whole U-Net's output and
+gradients against the CPU reference, and a few training steps."""
+
+import copy
+
+import pytest
+import torch
+import torch.nn.functional as Fn
+
+from ggems_amd import surrogate as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _ref_conv(x, w, b, relu):
+    """x [N,H,W,Cin], w [Cout,9,Cin] on the CPU in FP32 -> [N,H,W,Cout]"""
+    cout, _, cin = w.shape
+    y = Fn.conv2d(x.permute(0, 3, 1, 2), w.view(cout, 3, 3, cin).permute(0, 3, 1, 2), b, padding=1)
+    y = torch.relu(y) if relu else y
+    return y.permute(0, 2, 3, 1)
+
+
+@pytest.mark.parametrize("n,h,w,cin,cout,relu", [(2, 32, 32, 8, 16, False), (1, 64, 64, 16, 64, True),
+                                                 (2, 32, 64, 48, 128, True), (1, 32, 32, 192, 64, False)])
+def test_conv_forward_matches_fp32(n, h, w, cin, cout, relu):
+    g = torch.Generator().manual_seed(cin * cout)
+    x = torch.randn(n, h, w, cin, generator=g).bfloat16()
+    wt = (torch.randn(cout, 9, cin, generator=g) / (3 * cin ** 0.5)).bfloat16()
+    b = torch.randn(cout, generator=g)
+    y = S.conv3x3(x.cuda(), wt.view(cout, -1).cuda(), b.cuda(), relu).float().cpu()
+    ref = _ref_conv(x.float(), wt.float(), b, relu)
+    # FP32 accumulation, one BF16 rounding of the output
+    assert torch.all((y - ref).abs() <= 2 ** -7 * ref.abs() + 1e-5), float((y - ref).abs().max())
+
+
+@pytest.mark.parametrize("n,h,w,cin,cout", [(2, 32, 32, 16, 16), (1, 64, 64, 48, 128), (2, 16, 16, 128, 64)])
+def test_conv_gradients_match_fp32(n, h, w, cin, cout):
+    g = torch.Generator().manual_seed(7 + cin)
+    x = torch.randn(n, h, w, cin, generator=g).bfloat16()
+    wt = (torch.randn(cout, 9, cin, generator=g) / (3 * cin ** 0.5)).bfloat16()
+    dy = torch.randn(n, h, w, cout, generator=g).bfloat16()
+    xr = x.float().requires_grad_(True)
+    wr = wt.float().requires_grad_(True)
+    _ref_conv(xr, wr, None, False).backward(dy.float())
+    dw = S.conv3x3_weight_grad(x.cuda(), dy.cuda()).cpu().view(cout, 9, cin)
+    # FP32 sums of exact BF16 products, in another order
+    scale = wr.grad.abs().max()
+    assert torch.all((dw - wr.grad).abs() <= 1e-5 * scale + 1e-4 * wr.grad.abs()), float((dw - wr.grad).abs().max())
+    dx = S.conv3x3(dy.cuda(), S.flip_transpose(wt).view(cin, 9 * cout).cuda()).float().cpu()
+    assert torch.all((dx - xr.grad).abs() <= 2 ** -7 * xr.grad.abs() + 1e-4), float((dx - xr.grad).abs().max())
+
+
+def test_unet_forward_and_gradients_match_the_fp32_reference():
+    torch.manual_seed(0)
+    net = S.UNet(seed=5).cuda()
+    x = torch.rand(2, 128, 128, 2)
+    out = net(x.cuda())
+    target = torch.rand(2, 128, 128)
+    loss = ((out - target.cuda()) ** 2).mean()
+    loss.backward()
+    ref_net = copy.deepcopy(net).cpu().float()
+    for p in ref_net.parameters():
+        p.grad = None
+    ref = S.reference_forward(ref_net, x)
+    rel = float((out.detach().cpu() - ref.detach()).norm() / ref.detach().norm())
+    assert rel < 0.03, rel
+    ((ref - target) ** 2).mean().backward()
+    for (name, p), (_, q) in zip(net.named_parameters(), ref_net.named_parameters()):
+        a, b = p.grad.detach().cpu().reshape(-1), q.grad.reshape(-1)
+        if b.norm() == 0:
+            continue
+        cos = float(a @ b / (a.norm() * b.norm()))
+        assert cos > 0.99, (name, cos)
+        assert float((a - b).norm() / b.norm()) < 0.1, name
+
+
+def test_training_steps_reduce_the_loss():
+    torch.manual_seed(1)
+    net = S.UNet(seed=2).cuda()
+    opt = torch.optim.Adam(net.parameters(), lr=2e-3)
+    x = torch.rand(4, 128, 128, 2, device="cuda")
+    target = (x[..., 0] * 0.5 + torch.roll(x[..., 1], 3, dims=1) * 0.25).detach()
+    first = None
+    for step in range(60):
+        opt.zero_grad()
+        loss = ((net(x) - target) ** 2).mean()
+        loss.backward()
+        opt.step()
+        first = first if first is not None else float(loss)
+    assert float(loss) < 0.3 * first, (first, float(loss))
