@@ -174,6 +174,58 @@ def bench_acquisition(args):
     print(json.dumps(out), flush=True)
 
 
+BF16_PEAK_TFLOPS = 2500.0  # dense MFMA, MI355X
+
+
+def bench_surrogate(args):
+    """scatter surrogate: training views from the engine, U-Net training on
+    the MFMA convolutions, held-out error against the low-count Monte Carlo"""
+    import numpy as np
+    import torch
+
+    from ggems_amd import surrogate as S
+    wd = os.path.join(REPO, "ggems_amd", "data", "_gen", "surrogate")
+    t0 = time.perf_counter()
+    rate = S.open_field_rate(wd, args.n_high, seed=3)
+    xtr, ytr, _ = S.dataset(wd, args.train_views, 100, args.n_low, args.n_high, rate)
+    xte, yte, lte = S.dataset(wd, args.test_views, 100000, args.n_low, args.n_high, rate)
+    t_data = time.perf_counter() - t0
+    net, t_train, losses = S.train(xtr, ytr, steps=args.train_steps, batch=8, seed=1)
+    pred = S.predict(net, xte)
+    rms = lambda a: float(np.sqrt(np.mean(a.astype(np.float64) ** 2)))
+    scale = float(np.mean(yte))
+    flops_step = 3 * 8 * S.conv_flops(net, 256, 256)
+    # the MFMA convolution on its own: the widest decoder layer, batch 8
+    x = torch.randn(8, 64, 64, 192, device="cuda").bfloat16()
+    w = torch.randn(64, 9 * 192, device="cuda").bfloat16()
+    for _ in range(3):
+        S.conv3x3(x, w)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(20):
+        S.conv3x3(x, w)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 20
+    conv_tflops = 2 * 8 * 64 * 64 * 64 * 9 * 192 / (ms * 1e-3) / 1e12
+    out = {"metric": "scatter-surrogate RMS error on held-out views (relative to the mean scatter)",
+           "value": rms(pred - yte) / scale, "unit": "relative RMS", "n_gpus": 1, "higher_is_better": False,
+           "dtype": "bf16", "data": "synthetic",
+           "config": {"workload": "U-Net (4 levels, 16-32-64-128, 3x3 conv, BF16 MFMA, FP32 accumulate) on 256x256 "
+                                  "views of seeded 96x96x64 body phantoms (2 mm), 120 kVp; inputs: %d-primary MC "
+                                  "projection and the ray-traced transmission; target: %d-primary MC scatter"
+                                  % (args.n_low, args.n_high),
+                      "train_views": args.train_views, "test_views": args.test_views, "train_steps": args.train_steps},
+           "low_count_mc_relative_rms": rms(lte - yte) / scale,
+           "train_seconds": t_train, "data_seconds": t_data,
+           "train_conv_tflops": flops_step * args.train_steps / t_train / 1e12,
+           "losses": losses[::max(1, len(losses) // 8)],
+           "roofline": {"bound": "mfma", "achieved": conv_tflops, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": conv_tflops / BF16_PEAK_TFLOPS, "kernel": "conv_fwd_kernel",
+                        "shape": "8x64x64, 192 -> 64 channels", "kernel_ms": ms}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,7 +246,12 @@ def main():
     ap.add_argument("--general-kernel", action="store_true", help="run the general kernel variant")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run a plain device copy of this size (calibrates profiler byte counters)")
-    ap.add_argument("--workload", choices=("voxel_ct", "mesh_ct", "acquisition"), default="voxel_ct")
+    ap.add_argument("--workload", choices=("voxel_ct", "mesh_ct", "acquisition", "surrogate"), default="voxel_ct")
+    ap.add_argument("--n-low", type=int, default=1_000_000, help="surrogate: primaries of the input projection")
+    ap.add_argument("--n-high", type=int, default=100_000_000, help="surrogate: primaries of the target")
+    ap.add_argument("--train-views", type=int, default=48)
+    ap.add_argument("--test-views", type=int, default=12)
+    ap.add_argument("--train-steps", type=int, default=1500)
     ap.add_argument("--angles", type=int, default=360, help="acquisition: projections per phantom")
     ap.add_argument("--budget-scale", type=float, default=0.1,
                     help="acquisition: budgets uniform in [1e6, 3e7] times this")
@@ -208,6 +265,8 @@ def main():
         return bench_mesh_ct(args)
     if args.workload == "acquisition":
         return bench_acquisition(args)
+    if args.workload == "surrogate":
+        return bench_surrogate(args)
 
     import torch
 

@@ -308,6 +308,43 @@ def acquisition(workdir, side=128, seed=1):
     return [phantom, ct, src]
 
 
+def scatter_scene(workdir, phantom_seed=None, angle_deg=0.0, shape=(64, 96, 96), voxel=2.0, detector=DETECTOR):
+    """One view of the scatter-surrogate data: a seeded body phantom of
+    shape voxels (2 mm by default, body and ellipsoids scaled to the volume)
+    turned by angle_deg about z, the 120 kVp spectrum from a point source at
+    SID 600 mm and a detector of detector^2 1 mm GOS pixels at SDD 1000 mm
+    that stores the scatter image.  phantom_seed None: the open field."""
+    os.makedirs(workdir, exist_ok=True)
+    spec = os.path.join(workdir, "spectrum_120kVp_2.5mmAl.dat")
+    if not os.path.exists(spec):
+        GGEMSMaterialsDatabaseManager().set_materials(MATERIALS_TXT)
+        e, w = authoring.kramers_spectrum(120.0, 20.0, 2.5, authoring.aluminium_mu())
+        authoring.write_spectrum(spec, e, w)
+    keep = []
+    if phantom_seed is not None:
+        nz, ny, nx = shape
+        mhd = os.path.join(workdir, "body_%d_%dx%dx%d.mhd" % (phantom_seed, nx, ny, nz))
+        if not os.path.exists(mhd):
+            lab = authoring.ellipsoid_body_phantom(phantom_seed, shape=shape, voxel=voxel,
+                                                   body_axes=(0.45 * nx * voxel, 0.35 * ny * voxel),
+                                                   semi_axis_range=(nx * voxel / 50.0, nx * voxel / 5.0))
+            authoring.write_mhd(mhd, lab, (voxel, voxel, voxel))
+        rng = authoring.write_range(os.path.join(workdir, "range_body.txt"), authoring.BODY_MATERIALS)
+    setup(workdir)
+    if phantom_seed is not None:
+        phantom = GGEMSVoxelizedPhantom("phantom")
+        phantom.set_phantom(mhd, rng)
+        phantom.set_rotation(0.0, 0.0, angle_deg, "deg")
+        phantom.set_position(0.0, 0.0, 0.0, "mm")
+        keep.append(phantom)
+    keep.append(_detector(workdir, n=detector))
+    _common_physics()
+    src = _source(1, detector=detector)
+    src.set_polyenergy(spec)
+    keep.append(src)
+    return keep
+
+
 def run(seed=777):
     g = GGEMS()
     g.initialize(seed)

@@ -227,32 +227,39 @@ struct Ray {
 // (tmax - tmin) < 2*tol is evaluated in FP64 by the reference; 2*kTol is
 // exactly representable in FP32, so the FP32 comparison is identical.
 __host__ __device__ inline float distance_to_aabb(f3 p, const Ray& r, const Box& b) {
-  // bitwise & and | on the tests keep the evaluation free of branches
+  // Each axis gives the interval of t inside its slab.  An axis the ray is
+  // parallel to (|d| < EPSILON6) keeps the running interval when the point
+  // lies within the slab and empties it otherwise - the reference's early
+  // return.  Its "tmin > tmax" returns after the second and third axes
+  // reduce to one test at the end, since tmin only grows and tmax only
+  // shrinks.  All of it is selects on floats: no branches, no lane masks
+  // kept in registers.
+  const float inf = __builtin_inff();
   bool par = fabsf(r.d.x) < kEps6;
+  bool in = (p.x >= b.lo.x) & (p.x <= b.hi.x);
   float t1 = (b.lo.x - p.x) * r.inv.x, t2 = (b.hi.x - p.x) * r.inv.x;
-  float lo = t1 > t2 ? t2 : t1, hi = t1 > t2 ? t1 : t2;
-  float tmin = par ? kFltMin : lo;
-  float tmax = par ? kOutOfWorld : hi;
-  bool out = par & ((p.x < b.lo.x) | (p.x > b.hi.x));
+  float tmin = par ? (in ? kFltMin : inf) : (t1 > t2 ? t2 : t1);
+  float tmax = par ? (in ? kOutOfWorld : -inf) : (t1 > t2 ? t1 : t2);
 
   par = fabsf(r.d.y) < kEps6;
+  in = (p.y >= b.lo.y) & (p.y <= b.hi.y);
   t1 = (b.lo.y - p.y) * r.inv.y;
   t2 = (b.hi.y - p.y) * r.inv.y;
-  lo = t1 > t2 ? t2 : t1;
-  hi = t1 > t2 ? t1 : t2;
-  tmin = (!par & (lo > tmin)) ? lo : tmin;
-  tmax = (!par & (hi < tmax)) ? hi : tmax;
-  out |= par ? ((p.y < b.lo.y) | (p.y > b.hi.y)) : (tmin > tmax);
+  float lo = par ? (in ? -inf : inf) : (t1 > t2 ? t2 : t1);
+  float hi = par ? (in ? inf : -inf) : (t1 > t2 ? t1 : t2);
+  tmin = lo > tmin ? lo : tmin;
+  tmax = hi < tmax ? hi : tmax;
 
   par = fabsf(r.d.z) < kEps6;
+  in = (p.z >= b.lo.z) & (p.z <= b.hi.z);
   t1 = (b.lo.z - p.z) * r.inv.z;
   t2 = (b.hi.z - p.z) * r.inv.z;
-  lo = t1 > t2 ? t2 : t1;
-  hi = t1 > t2 ? t1 : t2;
-  tmin = (!par & (lo > tmin)) ? lo : tmin;
-  tmax = (!par & (hi < tmax)) ? hi : tmax;
-  out |= par ? ((p.z < b.lo.z) | (p.z > b.hi.z)) : (tmin > tmax);
+  lo = par ? (in ? -inf : inf) : (t1 > t2 ? t2 : t1);
+  hi = par ? (in ? inf : -inf) : (t1 > t2 ? t1 : t2);
+  tmin = lo > tmin ? lo : tmin;
+  tmax = hi < tmax ? hi : tmax;
 
+  bool out = tmin > tmax;
   out |= (tmin < 0.0f) & (tmax <= 0.0f);   // tmax < 0 || tmax == 0
   out |= (tmax - tmin) < 2.0f * kTol;
   const float d = tmin <= kEps6 ? tmax : tmin;

@@ -200,3 +200,112 @@ def reference_forward(net, x):
     for (a, b), s in zip(net.dec, reversed(skips)):
         z = conv(b, conv(a, torch.cat([Fn.interpolate(z, scale_factor=2, mode="nearest"), s], dim=1)))
     return torch.einsum("nchw,c->nhw", z, net.head) + net.head_bias
+
+
+# ---------------------------------------------------------------------------
+# training data from the engine, training, evaluation
+# ---------------------------------------------------------------------------
+def _view_engine(keep_detector_solid, seed, detector):
+    from .ggems import GGEMS
+    g = GGEMS()
+    g.initialize(seed)
+    eng = g.engine()
+    eng.box_elements[keep_detector_solid] = (detector, detector, True)
+    return g, eng
+
+
+def open_field_rate(workdir, n, seed=1, detector=256):
+    """Counts per primary of every pixel with no phantom (the normaliser)."""
+    from . import configs
+    keep = configs.scatter_scene(workdir, None, detector=detector)
+    g, eng = _view_engine(0, seed, detector)
+    eng.run(n)
+    counts, _ = eng.histogram(0)
+    g.delete()
+    del keep
+    return counts.astype(np.float64) / n
+
+
+def make_view(workdir, phantom_seed, angle_deg, n_low, n_high, rate, seed, detector=256, shape=(64, 96, 96)):
+    """(inputs [H, W, 2], target [H, W], low-count scatter estimate [H, W])
+    for one view: inputs = low-count counts / open-field counts and the
+    ray-traced transmission; target = high-count scatter / open-field."""
+    from . import configs
+    keep = configs.scatter_scene(workdir, phantom_seed, angle_deg, shape=shape, detector=detector)
+    g, eng = _view_engine(1, seed, detector)
+    norm = np.maximum(rate, 1e-12)
+    eng.run(n_low)
+    low, low_scatter = eng.histogram(1)
+    eng.reset_tallies()
+    eng.run(n_high)
+    _, high_scatter = eng.histogram(1)
+    drr = eng.drr(1)
+    g.delete()
+    del keep
+    x = np.stack([low / (norm * n_low), drr], axis=-1).astype(np.float32)
+    y = (high_scatter / (norm * n_high)).astype(np.float32)
+    return x, y, (low_scatter / (norm * n_low)).astype(np.float32)
+
+
+def dataset(workdir, n_views, seed, n_low, n_high, rate, detector=256, shape=(64, 96, 96)):
+    """n_views views of seeded phantoms (seed + k) at seeded random angles"""
+    rng = np.random.default_rng(seed)
+    xs, ys, ls = [], [], []
+    for k in range(n_views):
+        x, y, l = make_view(workdir, seed + k, float(rng.uniform(0.0, 360.0)), n_low, n_high, rate,
+                            seed=1000 + seed + k, detector=detector, shape=shape)
+        xs.append(x)
+        ys.append(y)
+        ls.append(l)
+    return np.stack(xs), np.stack(ys), np.stack(ls)
+
+
+def conv_flops(net, h, w):
+    """multiply-add FLOPs of the 3x3 convolutions of one forward pass"""
+    total, size = 0, (h, w)
+    for i, (a, b) in enumerate(net.enc):
+        for m in (a, b):
+            cout, _, cin = m.weight.shape
+            total += 2 * size[0] * size[1] * cout * 9 * cin
+        if i + 1 < len(net.enc):
+            size = (size[0] // 2, size[1] // 2)
+    for a, b in net.dec:
+        size = (size[0] * 2, size[1] * 2)
+        for m in (a, b):
+            cout, _, cin = m.weight.shape
+            total += 2 * size[0] * size[1] * cout * 9 * cin
+    return total
+
+
+def train(x, y, steps=1500, batch=8, lr=2e-3, seed=0, device="cuda"):
+    """Adam on the mean squared error; returns (net, seconds, losses)"""
+    import time
+    torch.manual_seed(seed)
+    net = UNet(seed=seed).to(device)
+    opt = torch.optim.Adam(net.parameters(), lr=lr)
+    sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, steps)
+    X = torch.from_numpy(x).to(device)
+    Y = torch.from_numpy(y).to(device)
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    losses = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for step in range(steps):
+        idx = torch.randint(0, X.shape[0], (batch,), generator=g)
+        opt.zero_grad(set_to_none=True)
+        loss = ((net(X[idx]) - Y[idx]) ** 2).mean()
+        loss.backward()
+        opt.step()
+        sched.step()
+        if step % 50 == 0 or step == steps - 1:
+            losses.append(float(loss))
+    torch.cuda.synchronize()
+    return net, time.perf_counter() - t0, losses
+
+
+@torch.no_grad()
+def predict(net, x, batch=8):
+    out = []
+    for i in range(0, x.shape[0], batch):
+        out.append(net(torch.from_numpy(x[i:i + batch]).to(next(net.parameters()).device)).float().cpu().numpy())
+    return np.concatenate(out)

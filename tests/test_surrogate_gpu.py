@@ -87,5 +87,5 @@ def test_training_steps_reduce_the_loss():
         loss = ((net(x) - target) ** 2).mean()
         loss.backward()
         opt.step()
-        first = first if first is not None else float(loss)
-    assert float(loss) < 0.3 * first, (first, float(loss))
+        first = first if first is not None else loss.item()
+    assert loss.item() < 0.3 * first, (first, loss.item())
