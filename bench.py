@@ -185,12 +185,16 @@ def bench_surrogate(args):
 
     from ggems_amd import surrogate as S
     wd = os.path.join(REPO, "ggems_amd", "data", "_gen", "surrogate")
+
+    def log(msg):
+        print("[surrogate] " + msg, file=sys.stderr, flush=True)
+
     t0 = time.perf_counter()
     rate = S.open_field_rate(wd, args.n_high, seed=3)
-    xtr, ytr, _ = S.dataset(wd, args.train_views, 100, args.n_low, args.n_high, rate)
-    xte, yte, lte = S.dataset(wd, args.test_views, 100000, args.n_low, args.n_high, rate)
+    xtr, ytr, _ = S.dataset(wd, args.train_views, 100, args.n_low, args.n_high, rate, log=log)
+    xte, yte, lte = S.dataset(wd, args.test_views, 100000, args.n_low, args.n_eval, rate, log=log)
     t_data = time.perf_counter() - t0
-    net, t_train, losses = S.train(xtr, ytr, steps=args.train_steps, batch=8, seed=1)
+    net, t_train, losses = S.train(xtr, ytr, steps=args.train_steps, batch=8, seed=1, log=log)
     pred = S.predict(net, xte)
     rms = lambda a: float(np.sqrt(np.mean(a.astype(np.float64) ** 2)))
     scale = float(np.mean(yte))
@@ -213,8 +217,9 @@ def bench_surrogate(args):
            "dtype": "bf16", "data": "synthetic",
            "config": {"workload": "U-Net (4 levels, 16-32-64-128, 3x3 conv, BF16 MFMA, FP32 accumulate) on 256x256 "
                                   "views of seeded 96x96x64 body phantoms (2 mm), 120 kVp; inputs: %d-primary MC "
-                                  "projection and the ray-traced transmission; target: %d-primary MC scatter"
-                                  % (args.n_low, args.n_high),
+                                  "projection and the ray-traced transmission; training targets: %d-primary MC "
+                                  "scatter; held-out targets: %d-primary MC scatter" % (args.n_low, args.n_high,
+                                                                                        args.n_eval),
                       "train_views": args.train_views, "test_views": args.test_views, "train_steps": args.train_steps},
            "low_count_mc_relative_rms": rms(lte - yte) / scale,
            "train_seconds": t_train, "data_seconds": t_data,
@@ -249,6 +254,7 @@ def main():
     ap.add_argument("--workload", choices=("voxel_ct", "mesh_ct", "acquisition", "surrogate"), default="voxel_ct")
     ap.add_argument("--n-low", type=int, default=1_000_000, help="surrogate: primaries of the input projection")
     ap.add_argument("--n-high", type=int, default=100_000_000, help="surrogate: primaries of the target")
+    ap.add_argument("--n-eval", type=int, default=1_000_000_000, help="surrogate: primaries of held-out targets")
     ap.add_argument("--train-views", type=int, default=48)
     ap.add_argument("--test-views", type=int, default=12)
     ap.add_argument("--train-steps", type=int, default=1500)

@@ -226,7 +226,8 @@ def open_field_rate(workdir, n, seed=1, detector=256):
     return counts.astype(np.float64) / n
 
 
-def make_view(workdir, phantom_seed, angle_deg, n_low, n_high, rate, seed, detector=256, shape=(64, 96, 96)):
+def make_view(workdir, phantom_seed, angle_deg, n_low, n_high, rate, seed, detector=256, shape=(64, 96, 96),
+              chunk=100_000_000):
     """(inputs [H, W, 2], target [H, W], low-count scatter estimate [H, W])
     for one view: inputs = low-count counts / open-field counts and the
     ray-traced transmission; target = high-count scatter / open-field."""
@@ -237,7 +238,10 @@ def make_view(workdir, phantom_seed, angle_deg, n_low, n_high, rate, seed, detec
     eng.run(n_low)
     low, low_scatter = eng.histogram(1)
     eng.reset_tallies()
-    eng.run(n_high)
+    done = 0
+    while done < n_high:  # tallies accumulate over the chunks
+        eng.run(min(chunk, n_high - done))
+        done += min(chunk, n_high - done)
     _, high_scatter = eng.histogram(1)
     drr = eng.drr(1)
     g.delete()
@@ -247,11 +251,13 @@ def make_view(workdir, phantom_seed, angle_deg, n_low, n_high, rate, seed, detec
     return x, y, (low_scatter / (norm * n_low)).astype(np.float32)
 
 
-def dataset(workdir, n_views, seed, n_low, n_high, rate, detector=256, shape=(64, 96, 96)):
+def dataset(workdir, n_views, seed, n_low, n_high, rate, detector=256, shape=(64, 96, 96), log=None):
     """n_views views of seeded phantoms (seed + k) at seeded random angles"""
     rng = np.random.default_rng(seed)
     xs, ys, ls = [], [], []
     for k in range(n_views):
+        if log and k % 8 == 0:
+            log("view %d / %d" % (k, n_views))
         x, y, l = make_view(workdir, seed + k, float(rng.uniform(0.0, 360.0)), n_low, n_high, rate,
                             seed=1000 + seed + k, detector=detector, shape=shape)
         xs.append(x)
@@ -277,7 +283,7 @@ def conv_flops(net, h, w):
     return total
 
 
-def train(x, y, steps=1500, batch=8, lr=2e-3, seed=0, device="cuda"):
+def train(x, y, steps=1500, batch=8, lr=2e-3, seed=0, device="cuda", log=None):
     """Adam on the mean squared error; returns (net, seconds, losses)"""
     import time
     torch.manual_seed(seed)
@@ -298,7 +304,9 @@ def train(x, y, steps=1500, batch=8, lr=2e-3, seed=0, device="cuda"):
         opt.step()
         sched.step()
         if step % 50 == 0 or step == steps - 1:
-            losses.append(float(loss))
+            losses.append(loss.item())
+            if log and step % 250 == 0:
+                log("step %d loss %.3e" % (step, losses[-1]))
     torch.cuda.synchronize()
     return net, time.perf_counter() - t0, losses
 
