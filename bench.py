@@ -7,8 +7,9 @@ source at SID 600 mm with a 120 kVp Kramers spectrum (2.5 mm Al), a flat
 256x256 detector of 1 mm GOS pixels at SDD 1000 mm, Compton + photoelectric +
 Rayleigh, FP64 dose + uncertainty tallies.  Set up through the GGEMS API
 (ggems_amd.configs.voxel_ct), then timed on the engine with every input
-resident in HBM: one step transports --photons-per-step primaries
-(batches of 1,048,576 particle slots, as GGEMS runs them).
+resident in HBM: one step is the workload's run of 1e9 primaries
+(--photons-per-step), one launch laid out in batches of 1,048,576 particle
+slots as GGEMS lays them out.
 
 Prints one JSON line (metric, value, roofline, cpu_baseline, ...).
 
@@ -234,9 +235,10 @@ def bench_surrogate(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--photons-per-step", type=int, default=1 << 24)
+    ap.add_argument("--photons-per-step", type=int, default=None,
+                    help="primaries per step (default: the workload's run, 1e9 for voxel_ct, 1e8 for mesh_ct)")
     ap.add_argument("--seed", type=int, default=777)
     ap.add_argument("--workdir", default=os.path.join(REPO, "ggems_amd", "data", "_gen", "voxel_ct"))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -267,6 +269,8 @@ def main():
               "particle slots, with one sum-reduction of the tallies at the end", file=sys.stderr)
         sys.exit(2)
 
+    if args.photons_per_step is None:
+        args.photons_per_step = 100_000_000 if args.workload == "mesh_ct" else 1_000_000_000
     if args.workload == "mesh_ct":
         return bench_mesh_ct(args)
     if args.workload == "acquisition":
