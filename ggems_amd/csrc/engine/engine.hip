@@ -1369,7 +1369,7 @@ struct ggh_context {
   DevBuf<uint32_t> queue;
   DevBuf<unsigned long long> stats;
   // launch shape
-  int hot_threshold = 32;
+  int hot_threshold = 40;
   int resume_threshold = 64;
   int interact_batch = 16;
   int blocks_per_cu = 0;  // 0: as many as fit
@@ -2055,7 +2055,7 @@ int ggh_set_launch(ggh_context* c, const ggh_launch_desc* d) {
   if (!d || d->hot_threshold < 0 || d->hot_threshold > 64 || d->interact_batch < 0 || d->interact_batch > 64 ||
       d->resume_threshold < 0 || d->resume_threshold > 64 || d->blocks_per_cu < 0 || d->blocks_per_cu > 64)
     return fail(-2, "invalid launch parameters");
-  c->hot_threshold = d->hot_threshold ? d->hot_threshold : 32;
+  c->hot_threshold = d->hot_threshold ? d->hot_threshold : 40;
   c->resume_threshold = std::max(d->resume_threshold ? d->resume_threshold : 64, c->hot_threshold);
   c->interact_batch = d->interact_batch ? d->interact_batch : 16;
   c->blocks_per_cu = d->blocks_per_cu;
@@ -2326,7 +2326,7 @@ int ggh_autotune(ggh_context* c, uint64_t n_trial, ggh_launch_desc* best) {
   float best_ms = INFINITY;
   ggh_launch_desc b{};
   for (int blocks : {1, 2})
-    for (int hot : {24, 32, 40})
+    for (int hot : {32, 40, 48})
       for (int batch : {8, 16, 32}) {
         c->blocks_per_cu = blocks;
         c->hot_threshold = hot;

@@ -9,10 +9,10 @@
 //
 // Forward (implicit GEMM, M = pixels, N = Cout, K = 9 Cin): a wave owns
 // 32 consecutive pixels (two runs of 16 within image rows) and 64 output
-// channels, i.e.
-// 2 x 4 accumulator tiles of 16 x 16; each K step of 32 loads one 16-byte
-// A fragment per pixel tile and one B fragment per channel tile and issues
-// 8 MFMAs.  Bias and ReLU are applied in the epilogue.  The data gradient
+// channels, i.e. 2 x 4 accumulator tiles of 16 x 16; each K step of 32 loads
+// one 16-byte A fragment per pixel tile from memory and one B fragment per
+// channel tile from the block's LDS copy of the weights, and issues 8 MFMAs.
+// Bias and ReLU are applied in the epilogue.  The data gradient
 // is the same kernel on the flipped, transposed weights.
 //
 // Weight gradient (M = Cout, N = 9 Cin, K = pixels): both operands have
@@ -60,18 +60,25 @@ __device__ inline bf16x8 load_a(const __bf16* x, int img, int y, int xx, int k0,
   return *reinterpret_cast<const bf16x8*>(x + (((size_t)img * H + yy) * W + xs) * Cin + ci);
 }
 
+// The block's four waves share one 64-channel tile and take four
+// consecutive 32-pixel tiles; each K step's weight chunk (64 channels x 32 k,
+// 4 KB) is staged once per block in LDS, double-buffered, instead of being
+// fetched from L2 by every wave.
+constexpr int kBRow = 32 + 8;  // bf16 per staged weight row (padded against bank conflicts)
+
 __global__ __launch_bounds__(64 * kWaves) void conv_fwd_kernel(const __bf16* __restrict__ x,
                                                                const __bf16* __restrict__ w,
                                                                const float* __restrict__ bias,
                                                                __bf16* __restrict__ y, int N, int H, int W, int Cin,
                                                                int Cout, int relu) {
+  __shared__ __bf16 sB[2][kFwdCh * kBRow];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int pix_tiles = (N * H * W) / kFwdPix;
   const int ch_tiles = (Cout + kFwdCh - 1) / kFwdCh;
-  const int job = blockIdx.x * kWaves + wave;
-  if (job >= pix_tiles * ch_tiles) return;
-  const int pt = job / ch_tiles, ct = job - pt * ch_tiles;
-  const int p0 = pt * kFwdPix;
+  const int ct = blockIdx.x % ch_tiles;
+  const int pt = (blockIdx.x / ch_tiles) * kWaves + wave;
+  const bool active = pt < pix_tiles;  // every wave stages and synchronises; only active ones compute
+  const int p0 = (active ? pt : 0) * kFwdPix;
   // each run of 16 pixels lies in one image row (W is a multiple of 16)
   int img[2], yrow[2], x0[2];
   for (int i = 0; i < 2; ++i) {
@@ -83,20 +90,31 @@ __global__ __launch_bounds__(64 * kWaves) void conv_fwd_kernel(const __bf16* __r
   const int c0 = ct * kFwdCh;
   const int K = 9 * Cin;
   const int r = lane & 15, g = lane >> 4;  // fragment row / k group
+  // staging: thread t copies 8 k of channel c0 + t / 4
+  const int s_co = threadIdx.x >> 2, s_kq = (threadIdx.x & 3) * 8;
+  auto stage = [&](int buf, int k0) {
+    const int co = c0 + s_co, k = k0 + s_kq;
+    bf16x8 v = (co < Cout && k < K) ? *reinterpret_cast<const bf16x8*>(w + (size_t)co * K + k) : zero8();
+    *reinterpret_cast<bf16x8*>(&sB[buf][s_co * kBRow + s_kq]) = v;
+  };
   f32x4 acc[2][4];
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  stage(0, 0);
+  __syncthreads();
+  int buf = 0;
   for (int k0 = 0; k0 < K; k0 += 32) {
+    if (k0 + 32 < K) stage(buf ^ 1, k0 + 32);  // the next chunk, while this one is used
     const int kk = k0 + 8 * g;
     bf16x8 a[2], b[4];
     for (int i = 0; i < 2; ++i) a[i] = load_a(x, img[i], yrow[i], x0[i] + r, kk, K, Cin, H, W);
-    for (int j = 0; j < 4; ++j) {
-      const int co = c0 + 16 * j + r;
-      b[j] = (co < Cout && kk < K) ? *reinterpret_cast<const bf16x8*>(w + (size_t)co * K + kk) : zero8();
-    }
+    for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const bf16x8*>(&sB[buf][(16 * j + r) * kBRow + 8 * g]);
     for (int i = 0; i < 2; ++i)
       for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+    buf ^= 1;
   }
+  if (!active) return;
   // C fragment: column r (output channel), rows 4 g .. 4 g + 3 (pixels)
   for (int j = 0; j < 4; ++j) {
     const int co = c0 + 16 * j + r;
@@ -109,6 +127,73 @@ __global__ __launch_bounds__(64 * kWaves) void conv_fwd_kernel(const __bf16* __r
         const int px = p0 + 16 * i + 4 * g + e;
         y[(size_t)px * Cout + co] = (__bf16)v;
       }
+  }
+}
+
+// Split-K variant for deep layers (K = 9 Cin of many 32-steps): the block's
+// four waves share one 32-pixel x 64-channel output tile and take every
+// fourth K chunk; a group of four chunks is staged in LDS at a time and the
+// four partial tiles are summed through LDS at the end.  A layer with few
+// pixel tiles gets four times the waves this way.
+__global__ __launch_bounds__(64 * kWaves) void conv_fwd_splitk_kernel(const __bf16* __restrict__ x,
+                                                                      const __bf16* __restrict__ w,
+                                                                      const float* __restrict__ bias,
+                                                                      __bf16* __restrict__ y, int N, int H, int W,
+                                                                      int Cin, int Cout, int relu) {
+  __shared__ __bf16 sB[kWaves][kFwdCh * kBRow];
+  __shared__ float sP[kWaves][32][64 + 1];  // partial tiles: [wave][element][lane]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ch_tiles = (Cout + kFwdCh - 1) / kFwdCh;
+  const int ct = blockIdx.x % ch_tiles, pt = blockIdx.x / ch_tiles;
+  const int p0 = pt * kFwdPix;
+  int img[2], yrow[2], x0[2];
+  for (int i = 0; i < 2; ++i) {
+    const int q = p0 + 16 * i, rem = q % (H * W);
+    img[i] = q / (H * W);
+    yrow[i] = rem / W;
+    x0[i] = rem - yrow[i] * W;
+  }
+  const int c0 = ct * kFwdCh;
+  const int K = 9 * Cin;
+  const int r = lane & 15, g = lane >> 4;
+  const int s_co = threadIdx.x >> 2, s_kq = (threadIdx.x & 3) * 8;
+  f32x4 acc[2][4];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int kg = 0; kg < K; kg += 32 * kWaves) {
+    // stage the group's four chunks: thread t copies 8 k of channel c0 + t / 4 of each
+    for (int c = 0; c < kWaves; ++c) {
+      const int co = c0 + s_co, k = kg + 32 * c + s_kq;
+      bf16x8 v = (co < Cout && k < K) ? *reinterpret_cast<const bf16x8*>(w + (size_t)co * K + k) : zero8();
+      *reinterpret_cast<bf16x8*>(&sB[c][s_co * kBRow + s_kq]) = v;
+    }
+    __syncthreads();
+    const int k0 = kg + 32 * wave;
+    if (k0 < K) {
+      const int kk = k0 + 8 * g;
+      bf16x8 a[2], b[4];
+      for (int i = 0; i < 2; ++i) a[i] = load_a(x, img[i], yrow[i], x0[i] + r, kk, K, Cin, H, W);
+      for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const bf16x8*>(&sB[wave][(16 * j + r) * kBRow + 8 * g]);
+      for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 4; ++j)
+      for (int e = 0; e < 4; ++e) sP[wave][(i * 4 + j) * 4 + e][lane] = acc[i][j][e];
+  __syncthreads();
+  // wave w sums and stores elements 8 w .. 8 w + 7 of every lane's tile
+  for (int q = 0; q < 8; ++q) {
+    const int idx = 8 * wave + q, i = idx >> 4, j = (idx >> 2) & 3, e = idx & 3;
+    const float sum = sP[0][idx][lane] + sP[1][idx][lane] + sP[2][idx][lane] + sP[3][idx][lane];
+    const int co = c0 + 16 * j + r;
+    if (co >= Cout) continue;
+    float v = sum + (bias ? bias[co] : 0.0f);
+    if (relu) v = v > 0.0f ? v : 0.0f;
+    const int px = p0 + 16 * i + 4 * g + e;
+    y[(size_t)px * Cout + co] = (__bf16)v;
   }
 }
 
@@ -195,10 +280,17 @@ int ggh_conv3x3_forward(const void* x, const void* w, const float* bias, void* y
                         int32_t cin, int32_t cout, int32_t relu, void* stream) {
   if (!x || !w || !y) return ufail(-1, "null pointer");
   if (!shapes_ok(n, h, wd, cin, cout)) return ufail(-2, "unsupported shape (W % 16, N H W % 32, Cin % 8, Cout % 16)");
-  const int jobs = (n * h * wd / kFwdPix) * ((cout + kFwdCh - 1) / kFwdCh);
-  hipLaunchKernelGGL(conv_fwd_kernel, dim3((jobs + kWaves - 1) / kWaves), dim3(64 * kWaves), 0,
-                     static_cast<hipStream_t>(stream), static_cast<const __bf16*>(x), static_cast<const __bf16*>(w),
-                     bias, static_cast<__bf16*>(y), n, h, wd, cin, cout, relu);
+  const int pix_tiles = n * h * wd / kFwdPix, ch_tiles = (cout + kFwdCh - 1) / kFwdCh;
+  // deep layers with few output tiles: split K over the block's waves
+  const bool split = 9 * cin >= 8 * 32 * kWaves && pix_tiles * ch_tiles < 1024;
+  if (split)
+    hipLaunchKernelGGL(conv_fwd_splitk_kernel, dim3(pix_tiles * ch_tiles), dim3(64 * kWaves), 0,
+                       static_cast<hipStream_t>(stream), static_cast<const __bf16*>(x),
+                       static_cast<const __bf16*>(w), bias, static_cast<__bf16*>(y), n, h, wd, cin, cout, relu);
+  else
+    hipLaunchKernelGGL(conv_fwd_kernel, dim3(((pix_tiles + kWaves - 1) / kWaves) * ch_tiles), dim3(64 * kWaves), 0,
+                       static_cast<hipStream_t>(stream), static_cast<const __bf16*>(x),
+                       static_cast<const __bf16*>(w), bias, static_cast<__bf16*>(y), n, h, wd, cin, cout, relu);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return ufail(-100 - (int)e, std::string("conv forward: ") + hipGetErrorString(e));
   return 0;

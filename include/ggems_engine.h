@@ -166,7 +166,7 @@ int ggh_get_stats(ggh_context* ctx, ggh_stats* out);
  * Zero fields keep the defaults. */
 typedef struct {
   int32_t hot_threshold;   /* a wave keeps stepping voxels while at least this many of its 64 lanes
-                              are inside a phantom (default 32) */
+                              are inside a phantom (default 40) */
   int32_t resume_threshold;/* the navigation phase ends once this many lanes step voxels again
                               (default 64: once every lane is back inside a phantom or done) */
   int32_t interact_batch;  /* inside the voxel loop, resolve pending Compton/Rayleigh/photoelectric
