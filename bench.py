@@ -192,13 +192,21 @@ def bench_surrogate(args):
 
     t0 = time.perf_counter()
     rate = S.open_field_rate(wd, args.n_high, seed=3)
-    xtr, ytr, _ = S.dataset(wd, args.train_views, 100, args.n_low, args.n_high, rate, log=log)
-    xte, yte, lte = S.dataset(wd, args.test_views, 100000, args.n_low, args.n_eval, rate, log=log)
+    tr = S.dataset(wd, args.train_views, 100, args.n_low, args.n_high, rate, log=log)
+    te = S.dataset(wd, args.test_views, 100000, args.n_low, args.n_eval, rate, log=log)
     t_data = time.perf_counter() - t0
-    net, t_train, losses = S.train(xtr, ytr, steps=args.train_steps, batch=8, seed=1, log=log)
-    pred = S.predict(net, xte)
-    rms = lambda a: float(np.sqrt(np.mean(a.astype(np.float64) ** 2)))
-    scale = float(np.mean(yte))
+    net, t_train, losses = S.train(tr["x"], tr["y"], steps=args.train_steps, batch=8, seed=1, log=log)
+    S.predict(net, te["x"][:8])  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pred = S.predict(net, te["x"])
+    t_infer = (time.perf_counter() - t0) / args.test_views
+    err = S.scatter_errors(pred, te["y"], te["var"], te["low"])
+    # wall time per view: the surrogate's inputs (low-count MC, ray trace) and
+    # its inference, against plain MC with enough primaries to match its error
+    t_sur = float(np.mean(te["t_low"] + te["t_drr"])) + t_infer
+    mc_rate = args.n_eval / float(np.mean(te["t_high"]))
+    t_mc = err["mc_primaries_factor"] * args.n_low / mc_rate
     flops_step = 3 * 8 * S.conv_flops(net, 256, 256)
     # the MFMA convolution on its own: the widest decoder layer, batch 8
     x = torch.randn(8, 64, 64, 192, device="cuda").bfloat16()
@@ -214,7 +222,7 @@ def bench_surrogate(args):
     ms = e0.elapsed_time(e1) / 20
     conv_tflops = 2 * 8 * 64 * 64 * 64 * 9 * 192 / (ms * 1e-3) / 1e12
     out = {"metric": "scatter-surrogate RMS error on held-out views (relative to the mean scatter)",
-           "value": rms(pred - yte) / scale, "unit": "relative RMS", "n_gpus": 1, "higher_is_better": False,
+           "value": err["pred"], "unit": "relative RMS", "n_gpus": 1, "higher_is_better": False,
            "dtype": "bf16", "data": "synthetic",
            "config": {"workload": "U-Net (4 levels, 16-32-64-128, 3x3 conv, BF16 MFMA, FP32 accumulate) on 256x256 "
                                   "views of seeded 96x96x64 body phantoms (2 mm), 120 kVp; inputs: %d-primary MC "
@@ -222,7 +230,15 @@ def bench_surrogate(args):
                                   "scatter; held-out targets: %d-primary MC scatter" % (args.n_low, args.n_high,
                                                                                         args.n_eval),
                       "train_views": args.train_views, "test_views": args.test_views, "train_steps": args.train_steps},
-           "low_count_mc_relative_rms": rms(lte - yte) / scale,
+           "low_count_mc_relative_rms": err["low"],
+           "target_noise_relative_rms": err["target_noise"],
+           "surrogate_vs_truth_relative_rms": err["pred_vs_truth"],
+           "low_count_mc_vs_truth_relative_rms": err["low_vs_truth"],
+           "mc_primaries_for_equal_error": err["mc_primaries_factor"] * args.n_low,
+           "seconds_per_view": {"surrogate": t_sur, "low_count_mc": float(np.mean(te["t_low"])),
+                                "ray_trace": float(np.mean(te["t_drr"])), "inference": t_infer,
+                                "plain_mc_at_equal_error": t_mc},
+           "wall_time_ratio_at_equal_error": t_mc / t_sur,
            "train_seconds": t_train, "data_seconds": t_data,
            "train_conv_tflops": flops_step * args.train_steps / t_train / 1e12,
            "losses": losses[::max(1, len(losses) // 8)],

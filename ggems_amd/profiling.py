@@ -15,6 +15,8 @@ Usage::
 
     python -m ggems_amd.profiling --kt D1 --fetch D2 --write D3 [--sq D4] \
         --copy-mb 256 --photons 16777216 --out summary.txt --json traffic.json
+
+Without ``--fetch``/``--write`` it summarises the kernel trace alone.
 """
 
 import argparse
@@ -57,7 +59,8 @@ def counter_values(path, counter, kernel):
 
 
 def top_kernels(path, n=6):
-    """(name, calls, total us, average us, percent) rows of the kernel-trace statistics."""
+    """(name, calls, total us, average us, percent) rows of the kernel-trace statistics (the top_kernels view
+    holds microseconds)."""
     con = _db(path)
     return con.execute("select name, total_calls, total_duration, average, percentage from top_kernels "
                        "order by total_duration desc limit ?", (n,)).fetchall()
@@ -82,35 +85,40 @@ def source_digest(repo=None):
     return h.hexdigest()
 
 
-def summarize(kt, fetch, write, sq=None, copy_mb=256, photons=None, kernel=KERNEL, extra=(), steps=None):
+def summarize(kt, fetch=None, write=None, sq=None, copy_mb=256, photons=None, kernel=KERNEL, extra=(), steps=None,
+              top=6):
+    """Kernel-trace figures of ``kernel``; HBM traffic too when both PMC runs are given."""
     copy_bytes = copy_mb * (1 << 20)
     calls, avg, lo, hi = kernel_durations(kt, kernel)
-    ff, fd, fkib = copy_factor(fetch, "FETCH_SIZE", copy_bytes)
-    wf, wd, wkib = copy_factor(write, "WRITE_SIZE", copy_bytes)
-    fetch_k = [v[2] for v in counter_values(fetch, "FETCH_SIZE", kernel)]
-    write_k = [v[2] for v in counter_values(write, "WRITE_SIZE", kernel)]
-    fetch_b = sum(fetch_k) / len(fetch_k) * 1024.0 * ff
-    write_b = sum(write_k) / len(write_k) * 1024.0 * wf
     out = {
         "kernel": kernel,
         "calls": calls,
         "avg_ns": avg,
         "min_ns": lo,
         "max_ns": hi,
-        "copy_bytes": copy_bytes,
-        "fetch_factor": ff,
-        "write_factor": wf,
-        "copy_fetch_kib": fkib,
-        "copy_write_kib": wkib,
-        "fetch_kib_raw": sum(fetch_k) / len(fetch_k),
-        "write_kib_raw": sum(write_k) / len(write_k),
-        "fetch_bytes": fetch_b,
-        "write_bytes": write_b,
-        "traffic_bytes": fetch_b + write_b,
         "photons": photons,
-        "top": top_kernels(kt),
+        "top": top_kernels(kt, top),
     }
-    if photons:
+    if fetch and write:
+        ff, fd, fkib = copy_factor(fetch, "FETCH_SIZE", copy_bytes)
+        wf, wd, wkib = copy_factor(write, "WRITE_SIZE", copy_bytes)
+        fetch_k = [v[2] for v in counter_values(fetch, "FETCH_SIZE", kernel)]
+        write_k = [v[2] for v in counter_values(write, "WRITE_SIZE", kernel)]
+        fetch_b = sum(fetch_k) / len(fetch_k) * 1024.0 * ff
+        write_b = sum(write_k) / len(write_k) * 1024.0 * wf
+        out.update({
+            "copy_bytes": copy_bytes,
+            "fetch_factor": ff,
+            "write_factor": wf,
+            "copy_fetch_kib": fkib,
+            "copy_write_kib": wkib,
+            "fetch_kib_raw": sum(fetch_k) / len(fetch_k),
+            "write_kib_raw": sum(write_k) / len(write_k),
+            "fetch_bytes": fetch_b,
+            "write_bytes": write_b,
+            "traffic_bytes": fetch_b + write_b,
+        })
+    if photons and "traffic_bytes" in out:
         out["traffic_bytes_per_primary"] = out["traffic_bytes"] / photons
     counters = {}
     for path in ([sq] if sq else []) + list(extra):
@@ -133,21 +141,24 @@ def format_summary(s, commands=()):
     lines.append("dispatches traced: %d   avg %.3f ms   min %.3f ms   max %.3f ms"
                  % (s["calls"], s["avg_ns"] * 1e-6, s["min_ns"] * 1e-6, s["max_ns"] * 1e-6))
     lines.append("kernel-trace statistics (all kernels of the traced run):")
-    lines.append("  %-60s %6s %12s %12s %7s" % ("name", "calls", "total ms", "avg ms", "%"))
+    lines.append("  %-72s %6s %12s %12s %7s" % ("name", "calls", "total ms", "avg ms", "%"))
     for name, calls, total, avg, pct in s["top"]:
-        lines.append("  %-60s %6d %12.3f %12.3f %7.3f" % (name[:60], calls, total * 1e-3, avg * 1e-3, pct))
-    lines.append("copy calibration (%d MiB device copy): FETCH_SIZE read %.1f KiB -> factor %.4f; "
-                 "WRITE_SIZE read %.1f KiB -> factor %.4f"
-                 % (s["copy_bytes"] >> 20, s["copy_fetch_kib"], s["fetch_factor"], s["copy_write_kib"],
-                    s["write_factor"]))
-    lines.append("FETCH_SIZE per dispatch: %.1f KiB raw -> %.3f GB calibrated"
-                 % (s["fetch_kib_raw"], s["fetch_bytes"] * 1e-9))
-    lines.append("WRITE_SIZE per dispatch: %.1f KiB raw -> %.3f GB calibrated"
-                 % (s["write_kib_raw"], s["write_bytes"] * 1e-9))
-    lines.append("traffic per dispatch: %.3f GB" % (s["traffic_bytes"] * 1e-9))
-    if s.get("photons"):
-        lines.append("primaries per dispatch: %d   traffic per primary: %.1f B"
-                     % (s["photons"], s["traffic_bytes_per_primary"]))
+        lines.append("  %-72s %6d %12.3f %12.3f %7.3f" % (name[:72], calls, total * 1e-3, avg * 1e-3, pct))
+    if "traffic_bytes" in s:
+        lines.append("copy calibration (%d MiB device copy): FETCH_SIZE read %.1f KiB -> factor %.4f; "
+                     "WRITE_SIZE read %.1f KiB -> factor %.4f"
+                     % (s["copy_bytes"] >> 20, s["copy_fetch_kib"], s["fetch_factor"], s["copy_write_kib"],
+                        s["write_factor"]))
+        lines.append("FETCH_SIZE per dispatch: %.1f KiB raw -> %.3f GB calibrated"
+                     % (s["fetch_kib_raw"], s["fetch_bytes"] * 1e-9))
+        lines.append("WRITE_SIZE per dispatch: %.1f KiB raw -> %.3f GB calibrated"
+                     % (s["write_kib_raw"], s["write_bytes"] * 1e-9))
+        lines.append("traffic per dispatch: %.3f GB" % (s["traffic_bytes"] * 1e-9))
+        if s.get("photons"):
+            lines.append("primaries per dispatch: %d   traffic per primary: %.1f B"
+                         % (s["photons"], s["traffic_bytes_per_primary"]))
+    elif s.get("photons"):
+        lines.append("primaries per dispatch: %d" % s["photons"])
     if s.get("sq"):
         lines.append("counters per dispatch:")
         for k in sorted(s["sq"]):
@@ -171,26 +182,29 @@ def format_summary(s, commands=()):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--kt", required=True)
-    ap.add_argument("--fetch", required=True)
-    ap.add_argument("--write", required=True)
+    ap.add_argument("--fetch", help="FETCH_SIZE run (with --write: HBM traffic per dispatch)")
+    ap.add_argument("--write", help="WRITE_SIZE run")
     ap.add_argument("--sq")
     ap.add_argument("--extra", action="append", default=[], help="further PMC run directories")
     ap.add_argument("--steps", type=float, help="voxel steps per dispatch (per-step instruction counts)")
     ap.add_argument("--copy-mb", type=int, default=256)
     ap.add_argument("--photons", type=int)
     ap.add_argument("--kernel", default=KERNEL)
+    ap.add_argument("--top", type=int, default=6, help="rows of the kernel-trace statistics to print")
     ap.add_argument("--out")
     ap.add_argument("--json")
     ap.add_argument("--command", action="append", default=[])
     args = ap.parse_args()
     s = summarize(args.kt, args.fetch, args.write, args.sq, args.copy_mb, args.photons, args.kernel, args.extra,
-                  args.steps)
+                  args.steps, args.top)
     text = format_summary(s, args.command)
     print(text, end="")
     if args.out:
         with open(args.out, "w") as f:
             f.write(text)
     if args.json:
+        if "traffic_bytes" not in s:
+            ap.error("--json needs --fetch and --write")
         rec = {"kernel": s["kernel"], "photons": s["photons"], "traffic_bytes": s["traffic_bytes"],
                "traffic_bytes_per_primary": s.get("traffic_bytes_per_primary"),
                "fetch_factor": s["fetch_factor"], "write_factor": s["write_factor"],

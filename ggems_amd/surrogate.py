@@ -173,7 +173,9 @@ class UNet(torch.nn.Module):
                 z = _pool(z)
         for (a, b), s in zip(self.dec, reversed(skips)):
             z = b(a(torch.cat([_up(z), s], dim=3)))
-        return z.float() @ self.head + self.head_bias
+        # the 16 -> 1 head as a multiply and a channel sum: a matrix-vector
+        # product with 16 columns runs far below HBM speed in the BLAS path
+        return (z.float() * self.head).sum(dim=3) + self.head_bias
 
 
 def reference_forward(net, x):
@@ -226,44 +228,94 @@ def open_field_rate(workdir, n, seed=1, detector=256):
     return counts.astype(np.float64) / n
 
 
+def _run_chunks(eng, n, chunk):
+    """n primaries in launches of at most `chunk`; the tallies accumulate"""
+    done = 0
+    while done < n:
+        m = min(chunk, n - done)
+        eng.run(m)
+        done += m
+
+
 def make_view(workdir, phantom_seed, angle_deg, n_low, n_high, rate, seed, detector=256, shape=(64, 96, 96),
               chunk=100_000_000):
-    """(inputs [H, W, 2], target [H, W], low-count scatter estimate [H, W])
-    for one view: inputs = low-count counts / open-field counts and the
-    ray-traced transmission; target = high-count scatter / open-field."""
+    """One view of a seeded phantom, as a dict:
+
+    * ``x`` [H, W, 2]: low-count counts / open-field counts, and the
+      ray-traced transmission (the network's inputs);
+    * ``y`` [H, W]: high-count scatter / open-field counts (the target);
+    * ``low`` [H, W]: the low-count run's own scatter image, same scale;
+    * ``var`` [H, W]: an unbiased estimate of the target's variance, from
+      the two independent halves of the high-count run:
+      var(y) = E[(y_a - y_b)^2] / 4 for equal halves;
+    * ``t_low``, ``t_drr``, ``t_high``: wall seconds of the low-count run,
+      of the ray trace and of the high-count run (each with its read-back).
+    """
+    import time
     from . import configs
     keep = configs.scatter_scene(workdir, phantom_seed, angle_deg, shape=shape, detector=detector)
     g, eng = _view_engine(1, seed, detector)
     norm = np.maximum(rate, 1e-12)
+    t0 = time.perf_counter()
     eng.run(n_low)
     low, low_scatter = eng.histogram(1)
-    eng.reset_tallies()
-    done = 0
-    while done < n_high:  # tallies accumulate over the chunks
-        eng.run(min(chunk, n_high - done))
-        done += min(chunk, n_high - done)
-    _, high_scatter = eng.histogram(1)
+    t_low = time.perf_counter() - t0
+    t0 = time.perf_counter()
     drr = eng.drr(1)
+    t_drr = time.perf_counter() - t0
+    eng.reset_tallies()
+    half = n_high // 2
+    t0 = time.perf_counter()
+    _run_chunks(eng, half, chunk)
+    _, scatter_a = eng.histogram(1)
+    _run_chunks(eng, n_high - half, chunk)
+    _, scatter = eng.histogram(1)
+    t_high = time.perf_counter() - t0
     g.delete()
     del keep
-    x = np.stack([low / (norm * n_low), drr], axis=-1).astype(np.float32)
-    y = (high_scatter / (norm * n_high)).astype(np.float32)
-    return x, y, (low_scatter / (norm * n_low)).astype(np.float32)
+    ya = scatter_a / (norm * half)
+    yb = (scatter - scatter_a) / (norm * (n_high - half))
+    return {"x": np.stack([low / (norm * n_low), drr], axis=-1).astype(np.float32),
+            "y": (scatter / (norm * n_high)).astype(np.float32),
+            "low": (low_scatter / (norm * n_low)).astype(np.float32),
+            "var": ((ya - yb) ** 2 / 4.0).astype(np.float32),
+            "t_low": t_low, "t_drr": t_drr, "t_high": t_high}
 
 
 def dataset(workdir, n_views, seed, n_low, n_high, rate, detector=256, shape=(64, 96, 96), log=None):
-    """n_views views of seeded phantoms (seed + k) at seeded random angles"""
+    """n_views views of seeded phantoms (seed + k) at seeded random angles:
+    the fields of `make_view`, stacked (arrays) or listed (timings)"""
     rng = np.random.default_rng(seed)
-    xs, ys, ls = [], [], []
+    views = []
     for k in range(n_views):
         if log and k % 8 == 0:
             log("view %d / %d" % (k, n_views))
-        x, y, l = make_view(workdir, seed + k, float(rng.uniform(0.0, 360.0)), n_low, n_high, rate,
-                            seed=1000 + seed + k, detector=detector, shape=shape)
-        xs.append(x)
-        ys.append(y)
-        ls.append(l)
-    return np.stack(xs), np.stack(ys), np.stack(ls)
+        views.append(make_view(workdir, seed + k, float(rng.uniform(0.0, 360.0)), n_low, n_high, rate,
+                               seed=1000 + seed + k, detector=detector, shape=shape))
+    return {k: (np.stack([v[k] for v in views]) if isinstance(views[0][k], np.ndarray) else
+                np.array([v[k] for v in views])) for k in views[0]}
+
+
+def scatter_errors(pred, y, var, low):
+    """Errors relative to the mean target, raw and against the truth.
+
+    The raw error compares with the noisy high-count target. Its expected
+    square is (error against the truth)^2 + var(y), so subtracting the
+    target's estimated variance gives the error against the noise-free
+    scatter. The low-count run's scatter is an unbiased estimate whose error
+    against the truth is its own noise, which falls as 1 / sqrt(primaries):
+    ``mc_primaries_factor`` is how many times more primaries than the
+    low-count run plain MC needs to match the prediction's error."""
+    mean = float(np.mean(y))
+    ms = lambda a: float(np.mean(a.astype(np.float64) ** 2))
+    noise = float(np.mean(var.astype(np.float64)))
+    pred_ms, low_ms = ms(pred - y), ms(low - y)
+    pred_true = max(pred_ms - noise, 0.0)
+    low_true = max(low_ms - noise, 0.0)
+    out = {"pred": np.sqrt(pred_ms) / mean, "low": np.sqrt(low_ms) / mean, "target_noise": np.sqrt(noise) / mean,
+           "pred_vs_truth": np.sqrt(pred_true) / mean, "low_vs_truth": np.sqrt(low_true) / mean}
+    out["mc_primaries_factor"] = low_true / pred_true if pred_true > 0 else float("inf")
+    return out
 
 
 def conv_flops(net, h, w):
