@@ -260,11 +260,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--hot-threshold", type=int, default=0, help="lanes a wave keeps stepping voxels for (0: engine default)")
+    ap.add_argument("--round-steps", type=int, default=0,
+                    help="voxel steps a wave makes at least between regroupings (0: engine default)")
     ap.add_argument("--interact-batch", type=int, default=0,
                     help="resolve interactions inside the voxel loop once this many lanes wait (0: engine default)")
-    ap.add_argument("--resume-threshold", type=int, default=0,
-                    help="lanes stepping voxels that end a navigation phase (0: engine default)")
+    ap.add_argument("--cold-passes", type=int, default=0,
+                    help="navigation passes per round for stopped photons (0: engine default)")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--general-kernel", action="store_true", help="run the general kernel variant")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -310,8 +311,8 @@ def main():
     g = GGEMS()
     g.initialize(args.seed)
     eng = g.engine()
-    eng.set_launch(args.hot_threshold, args.interact_batch, args.blocks_per_cu, args.general_kernel,
-                   args.resume_threshold)
+    eng.set_launch(args.round_steps, args.interact_batch, args.blocks_per_cu, args.general_kernel,
+                   args.cold_passes)
     for _ in range(args.warmup):
         eng.run(args.photons_per_step)
     eng.reset_tallies()

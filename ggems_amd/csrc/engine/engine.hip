@@ -13,16 +13,19 @@
 //
 // Divergence.  The voxel step is the hot path (hundreds of steps per
 // primary, about one interaction per hundred steps).  A lane's work is a
-// small state machine; a wave alternates
-//   * a hot phase: voxel steps while enough lanes are inside a phantom.  A
-//     step that ends in an interaction, an exit or the energy cut only
-//     records why the lane stopped, so the step itself stays short and
-//     branch-light;
-//   * a cold phase: every stopped lane resolves its interaction, leaves the
+// small state machine with two kinds of work:
+//   * voxel steps while the photon is inside a phantom.  A step that ends in
+//     an interaction, an exit or the energy cut only records why the lane
+//     stopped, so the step itself stays short and branch-light;
+//   * navigation for a stopped lane: it resolves its interaction, leaves the
 //     phantom, starts its next primary, crosses to the next solid or tracks
 //     through a detector module, until it is back inside a phantom.
-// The state machine only reorders work between lanes; each lane consumes
-// its stream in exactly the reference order.
+// A block works in rounds: a few navigation passes, voxel steps, then the
+// block regroups its photons through LDS so that the photons inside a
+// phantom fill whole waves and the stopped ones gather in a few others
+// (transport_kernel).  Regrouping and the state machine only reorder work
+// between lanes; each photon consumes its slot's stream in exactly the
+// reference order.
 #include <hip/hip_runtime.h>
 
 #include <fcntl.h>
@@ -131,9 +134,10 @@ struct RunParams {
   uint32_t n_batches, base, rem;
   uint32_t* queue;            // next slot to hand out
   unsigned long long* stats;  // ggh_stats fields
-  int32_t hot_threshold;
-  int32_t resume_threshold;
+  int32_t round_steps;        // voxel steps a wave makes at least per round
+  int32_t cold_passes;        // navigation passes per round
   int32_t interact_batch;
+  uint32_t xchg_offset;       // byte offset of the lane-state exchange area in dynamic LDS
   VoxGeom vox0;               // the only voxel solid, when there is exactly one
   // CT acquisition: projection p places the source at acq_src[p] and box
   // solid j at acq_box[p * n_box + j]; its primaries are laid out in
@@ -837,7 +841,7 @@ __device__ inline void next_projection(const RunParams& P, Lane& L) {
 // the cold work of one lane, one state at a time in an order that lets a
 // lane pass through several states in one call
 template <bool kSingle, bool kAcq>
-__device__ inline void cold_step(const RunParams& P, const Tabs& T, Lane& L, VoxGeom& G, Counters& c) {
+__device__ inline void cold_step(const RunParams& P, const Tabs& T, Lane& L, Counters& c) {
   if (L.mode == kInteract) resolve_interaction(P, T, L, c);
   if (L.mode == kCut) {
     if (L.solid == P.dose_solid) dose_record(P, L.e, L.p);
@@ -910,7 +914,6 @@ __device__ inline void cold_step(const RunParams& P, const Tabs& T, Lane& L, Vox
     if (s.type == kVoxel) {
       L.p = g2l_pos(s.m, L.p);
       L.ray.set(g2l_dir(s.m, L.ray.d));
-      if (!kSingle) G = s.vox;
       L.mode = kInVoxel;
     } else if (!kSingle && s.type == kMesh) {
       // ggh_run launches the single-voxel-solid variant only on scenes
@@ -923,12 +926,64 @@ __device__ inline void cold_step(const RunParams& P, const Tabs& T, Lane& L, Vox
   }
 }
 
+// Block-level regrouping of photons between waves.  A lane's whole state
+// moves through LDS to the position `pos` of the block, in two passes of 12
+// words; each pass reads back exactly the words it wrote, so the second
+// pass still writes this lane's own remaining words.  The permutation does
+// not change any history: a photon carries its slot, its random stream and
+// its counters of work left.
+constexpr int kWaves = kBlock / 64;
+constexpr size_t kXchgBytes = (size_t)kBlock * 12 * sizeof(uint32_t);
+
+__device__ inline void exchange_lane(Lane& L, uint4* s_x, uint32_t pos) {
+  const uint4* src = s_x + (size_t)threadIdx.x * 3;
+  uint4* dst = s_x + (size_t)pos * 3;
+  dst[0] = make_uint4(L.r.x, L.r.y, L.r.z, L.r.w);
+  dst[1] = make_uint4(L.r.c, __float_as_uint(L.p.x), __float_as_uint(L.p.y), __float_as_uint(L.p.z));
+  dst[2] = make_uint4(__float_as_uint(L.ray.d.x), __float_as_uint(L.ray.d.y), __float_as_uint(L.ray.d.z),
+                      __float_as_uint(L.ray.inv.x));
+  __syncthreads();
+  uint4 a = src[0], b = src[1], d = src[2];
+  L.r = Rng{a.x, a.y, a.z, a.w, b.x};
+  L.p = mk(__uint_as_float(b.y), __uint_as_float(b.z), __uint_as_float(b.w));
+  L.ray.d = mk(__uint_as_float(d.x), __uint_as_float(d.y), __uint_as_float(d.z));
+  L.ray.inv.x = __uint_as_float(d.w);
+  __syncthreads();
+  dst[0] = make_uint4(__float_as_uint(L.ray.inv.y), __float_as_uint(L.ray.inv.z), __float_as_uint(L.e),
+                      (uint32_t)L.eid);
+  dst[1] = make_uint4((uint32_t)L.mode, (uint32_t)L.solid, (uint32_t)L.hop, (uint32_t)L.scatter);
+  dst[2] = make_uint4((uint32_t)L.pend, L.todo, L.slot, L.proj);
+  __syncthreads();
+  a = src[0];
+  b = src[1];
+  d = src[2];
+  L.ray.inv.y = __uint_as_float(a.x);
+  L.ray.inv.z = __uint_as_float(a.y);
+  L.e = __uint_as_float(a.z);
+  L.eid = (int32_t)a.w;
+  L.mode = (int32_t)b.x;
+  L.solid = (int32_t)b.y;
+  L.hop = (int32_t)b.z;
+  L.scatter = (int32_t)b.w;
+  L.pend = (int32_t)d.x;
+  L.todo = d.y;
+  L.slot = d.z;
+  L.proj = d.w;
+  __syncthreads();
+}
+
+__device__ inline uint32_t lds_load_relaxed(uint32_t* p) {
+  return __builtin_amdgcn_readfirstlane(__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+}
+
 // NP: activated processes (-1: read at run time); kSingle: the scene has
 // exactly one voxel solid, whose geometry the voxel step reads from the
 // kernel arguments, and no meshed solid; kLDS: step tables staged in LDS; kTrack: photon
 // fluence tracking may be on; kAcq: a CT acquisition (per-projection source
-// and module placement).
-template <int NP, bool kSingle, bool kLDS, bool kTrack, bool kAcq>
+// and module placement); kRegroup: the scene has a voxel solid, and blocks
+// work in rounds with regrouping (below).  Without a voxel solid every
+// photon is navigated and tracked by its lane's state machine alone.
+template <int NP, bool kSingle, bool kLDS, bool kTrack, bool kAcq, bool kRegroup>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWavesPerEU)))
 void transport_kernel(RunParams P) {
   extern __shared__ float4 s_tab[];
@@ -980,36 +1035,99 @@ void transport_kernel(RunParams P) {
   L.todo = 0;
   L.slot = 0xffffffffu;
   L.proj = 0;
-  VoxGeom G = P.vox0;
   Counters c = {0, 0, 0, 0, 0};
   unsigned long long steps = 0;  // wave-uniform
 
-  for (;;) {
-    // ---- cold phase: every stopped lane advances until it is back inside
-    //      a phantom or has no work left,
-    //      or until enough lanes are stepping again (the rest continue in
-    //      the next navigation phase)
+  if (!kRegroup) {
+    // no voxel solid, so no lane ever steps voxels: the loop ends when no
+    // lane has navigation or tracking left
     for (;;) {
       const bool cold = L.mode != kInVoxel && L.mode != kDone;
-      if (!__any(cold) || __popcll(__ballot(L.mode == kInVoxel)) >= P.resume_threshold) break;
-      if (cold) cold_step<kSingle, kAcq>(P, T, L, G, c);
+      if (!__any(cold)) break;
+      if (cold) cold_step<kSingle, kAcq>(P, T, L, c);
     }
-    // ---- hot phase: voxel steps while enough lanes are inside a phantom
-    uint64_t act = __ballot(L.mode == kInVoxel);
-    if (act == 0) break;  // every lane is done
-    const int live = __popcll(__ballot(L.mode != kDone));
-    const int need = min(P.hot_threshold, live);
-    do {
-      bool stepped = false;
-      if (L.mode == kInVoxel) stepped = hot_step<NP, kTrack>(P, T, kSingle ? P.vox0 : G, L);
-      steps += __popcll(__ballot(stepped));
-      // pending interactions resolved in batches, so that scattered photons
-      // keep stepping without waiting for the navigation phase
-      if (__popcll(__ballot(L.mode == kInteract)) >= P.interact_batch) {
-        if (L.mode == kInteract) resolve_interaction(P, T, L, c);
+  } else {
+    // The block works in rounds.  In each round every wave
+    //   1. advances its stopped lanes (interaction, exit, detector, new
+    //      primary or slot) by at most cold_passes navigation passes;
+    //   2. makes voxel steps until every wave of the block is ready to stop:
+    //      a wave is ready once all waves have finished step 1 and it has made
+    //      round_steps steps (or has no lane inside a phantom); ready waves
+    //      keep stepping until all are, so no wave idles at the barrier
+    //      longer than about one step;
+    //   3. regroups the block's photons: those inside a phantom first, then
+    //      the stopped ones, then lanes with no work left.
+    // Step 3 fills waves with photons in the same phase, so that voxel steps
+    // run on full waves and the navigation code runs once for many photons.
+    // Every wave passes the same barriers in every round, and every loop
+    // above is bounded, so the block cannot deadlock.
+    uint4* s_x = reinterpret_cast<uint4*>(reinterpret_cast<char*>(s_tab) + P.xchg_offset);
+    __shared__ uint32_t s_cnt[kWaves][2];
+    __shared__ uint32_t s_cold_done[2];
+    __shared__ uint32_t s_ready[2];
+    const int wave = (int)(threadIdx.x >> 6), lane = __lane_id();
+    const uint64_t below = (1ull << lane) - 1ull;
+    if (threadIdx.x < 2) {
+      s_cold_done[threadIdx.x] = 0;
+      s_ready[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    for (uint32_t round = 0;; ++round) {
+      const uint32_t rb = round & 1u;
+      for (int pass = 0; pass < P.cold_passes; ++pass) {
+        const bool cold = L.mode != kInVoxel && L.mode != kDone;
+        if (!__any(cold)) break;
+        if (cold) cold_step<kSingle, kAcq>(P, T, L, c);
       }
-      act = __ballot(L.mode == kInVoxel);
-    } while (__popcll(act) >= need);
+      if (lane == 0) atomicAdd(&s_cold_done[rb], 1u);
+      bool ready = false;
+      for (int k = 0;; ++k) {
+        const uint64_t act = __ballot(L.mode == kInVoxel);
+        if (!ready && (act == 0 || (k >= P.round_steps && lds_load_relaxed(&s_cold_done[rb]) == (uint32_t)kWaves))) {
+          ready = true;
+          if (lane == 0) atomicAdd(&s_ready[rb], 1u);
+        }
+        if (act == 0 || (ready && lds_load_relaxed(&s_ready[rb]) == (uint32_t)kWaves)) break;
+        bool stepped = false;
+        // the phantom's geometry: kernel arguments for the only voxel
+        // solid, else the lane's solid in the (LDS-staged) solid table
+        if (L.mode == kInVoxel) stepped = hot_step<NP, kTrack>(P, T, kSingle ? P.vox0 : T.solids[L.solid].vox, L);
+        steps += __popcll(__ballot(stepped));
+        // pending interactions resolved in batches, so that scattered photons
+        // keep stepping without waiting for the next round
+        if (__popcll(__ballot(L.mode == kInteract)) >= P.interact_batch) {
+          if (L.mode == kInteract) resolve_interaction(P, T, L, c);
+        }
+      }
+      // regrouping: rank 0 inside a phantom, 1 stopped, 2 no work left
+      const int key = L.mode == kInVoxel ? 0 : (L.mode != kDone ? 1 : 2);
+      const uint64_t b0 = __ballot(key == 0), b1 = __ballot(key == 1);
+      if (lane == 0) {
+        s_cnt[wave][0] = (uint32_t)__popcll(b0);
+        s_cnt[wave][1] = (uint32_t)__popcll(b1);
+      }
+      __syncthreads();
+      uint32_t n_hot = 0, n_cold = 0, before0 = 0, before1 = 0, before2 = 0;
+      for (int w = 0; w < kWaves; ++w) {
+        const uint32_t h = s_cnt[w][0], cl = s_cnt[w][1];
+        n_hot += h;
+        n_cold += cl;
+        if (w < wave) {
+          before0 += h;
+          before1 += cl;
+          before2 += 64u - h - cl;
+        }
+      }
+      if (n_hot + n_cold == 0) break;  // read from LDS after a barrier: the same for the whole block
+      if (threadIdx.x == 0) {  // the next round's counters; nobody reads them before the barriers below
+        s_cold_done[rb ^ 1u] = 0;
+        s_ready[rb ^ 1u] = 0;
+      }
+      const uint64_t mine = key == 0 ? b0 : (key == 1 ? b1 : ~(b0 | b1));
+      const uint32_t pos = (key == 0 ? before0 : (key == 1 ? n_hot + before1 : n_hot + n_cold + before2)) +
+                           (uint32_t)__popcll(mine & below);
+      exchange_lane(L, s_x, pos);
+    }
   }
 
   uint32_t sums[5] = {c.primaries, c.interactions, c.detector_steps, c.detector_interactions, c.hops};
@@ -1369,8 +1487,8 @@ struct ggh_context {
   DevBuf<uint32_t> queue;
   DevBuf<unsigned long long> stats;
   // launch shape
-  int hot_threshold = 40;
-  int resume_threshold = 64;
+  int round_steps = 20;
+  int cold_passes = 2;
   int interact_batch = 16;
   int blocks_per_cu = 0;  // 0: as many as fit
   bool force_general = false;
@@ -1477,9 +1595,10 @@ RunParams make_params(ggh_context* c) {
   P.rng_stride = GGH_MAX_SLOTS;
   P.queue = c->queue.p;
   P.stats = c->stats.p;
-  P.hot_threshold = c->hot_threshold;
-  P.resume_threshold = c->resume_threshold;
+  P.round_steps = c->round_steps;
+  P.cold_passes = c->cold_passes;
   P.interact_batch = c->interact_batch;
+  P.xchg_offset = 0;
   for (const DevSolid& s : c->solids)
     if (s.type == kVoxel) {
       P.vox0 = s.vox;
@@ -1565,25 +1684,36 @@ struct Launch {
   void (*fn)(RunParams);
   unsigned grid;
   size_t shmem;
+  uint32_t xchg_offset;  // RunParams::xchg_offset of this launch
 };
 
 // kernel variant and persistent grid: the specialised variant (three
 // activated processes, one voxel solid and no mesh, tables in LDS, no fluence
-// tracking) or the general one, each with or without acquisition placement
+// tracking) or the general one, each with or without acquisition placement;
+// block regrouping whenever the scene has a voxel solid to step through
 int pick_launch(ggh_context* c, uint64_t slots, bool acq, Launch& L) {
   const bool single = n_solids_of(c, kVoxel) == 1 && n_solids_of(c, kMesh) == 0;
   const size_t lds = tables_lds_bytes(c);
   const bool use_lds = lds <= (size_t)kLdsBudget;
-  L.shmem = use_lds ? lds : 0;
   const bool special = c->n_proc == 3 && single && use_lds && !c->ptrack.p && !c->force_general;
-  if (acq)
-    L.fn = special ? transport_kernel<3, true, true, false, true>
-                   : (use_lds ? transport_kernel<-1, false, true, true, true>
-                              : transport_kernel<-1, false, false, true, true>);
+  const bool regroup = n_solids_of(c, kVoxel) > 0;
+  // dynamic LDS: the staged tables (when they fit), then the lane-state
+  // exchange area of the block's regrouping
+  L.xchg_offset = use_lds ? (uint32_t)((lds + 15) & ~(size_t)15) : 0u;
+  L.shmem = L.xchg_offset + (regroup ? kXchgBytes : 0);
+  if (special)
+    L.fn = acq ? transport_kernel<3, true, true, false, true, true>
+               : transport_kernel<3, true, true, false, false, true>;
+  else if (acq)
+    L.fn = use_lds ? (regroup ? transport_kernel<-1, false, true, true, true, true>
+                              : transport_kernel<-1, false, true, true, true, false>)
+                   : (regroup ? transport_kernel<-1, false, false, true, true, true>
+                              : transport_kernel<-1, false, false, true, true, false>);
   else
-    L.fn = special ? transport_kernel<3, true, true, false, false>
-                   : (use_lds ? transport_kernel<-1, false, true, true, false>
-                              : transport_kernel<-1, false, false, true, false>);
+    L.fn = use_lds ? (regroup ? transport_kernel<-1, false, true, true, false, true>
+                              : transport_kernel<-1, false, true, true, false, false>)
+                   : (regroup ? transport_kernel<-1, false, false, true, false, true>
+                              : transport_kernel<-1, false, false, true, false, false>);
   c->last_variant = special ? 0 : 1;
   int per_cu = 0;
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(L.fn), kBlock,
@@ -1623,7 +1753,9 @@ int acquisition_launch(ggh_context* c, RunParams& P, Launch& L) {
   P.n_proj = c->acq_n_proj;
   P.n_box = n_solids_of(c, kBoxSolid);
   P.n_slots = c->acq_slots;
-  return pick_launch(c, c->acq_slots, true, L);
+  if ((rc = pick_launch(c, c->acq_slots, true, L))) return rc;
+  P.xchg_offset = L.xchg_offset;
+  return 0;
 }
 
 }  // namespace
@@ -2052,11 +2184,11 @@ int ggh_reset_tallies(ggh_context* c) {
 int ggh_set_launch(ggh_context* c, const ggh_launch_desc* d) {
   int rc = check_ctx(c);
   if (rc) return rc;
-  if (!d || d->hot_threshold < 0 || d->hot_threshold > 64 || d->interact_batch < 0 || d->interact_batch > 64 ||
-      d->resume_threshold < 0 || d->resume_threshold > 64 || d->blocks_per_cu < 0 || d->blocks_per_cu > 64)
+  if (!d || d->round_steps < 0 || d->round_steps > 1024 || d->cold_passes < 0 || d->cold_passes > 16 ||
+      d->interact_batch < 0 || d->interact_batch > 64 || d->blocks_per_cu < 0 || d->blocks_per_cu > 64)
     return fail(-2, "invalid launch parameters");
-  c->hot_threshold = d->hot_threshold ? d->hot_threshold : 40;
-  c->resume_threshold = std::max(d->resume_threshold ? d->resume_threshold : 64, c->hot_threshold);
+  c->round_steps = d->round_steps ? d->round_steps : 20;
+  c->cold_passes = d->cold_passes ? d->cold_passes : 2;
   c->interact_batch = d->interact_batch ? d->interact_batch : 16;
   c->blocks_per_cu = d->blocks_per_cu;
   c->force_general = d->general_kernel != 0;
@@ -2077,6 +2209,7 @@ int ggh_run(ggh_context* c, uint64_t n) {
   P.n_slots = slots;
   Launch L;
   if ((rc = pick_launch(c, slots, false, L))) return rc;
+  P.xchg_offset = L.xchg_offset;
   HIP_TRY(hipMemsetAsync(c->queue.p, 0, sizeof(uint32_t), c->stream));
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
   hipLaunchKernelGGL(L.fn, dim3(L.grid), dim3(kBlock), L.shmem, c->stream, P);
@@ -2321,23 +2454,22 @@ int ggh_autotune(ggh_context* c, uint64_t n_trial, ggh_launch_desc* best) {
   DevBuf<uint32_t> saved;
   if ((rc = dev_alloc(saved, c->rng.n))) return rc;
   HIP_TRY(hipMemcpyAsync(saved.p, c->rng.p, c->rng.n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
-  const int keep_hot = c->hot_threshold, keep_res = c->resume_threshold, keep_batch = c->interact_batch,
+  const int keep_round = c->round_steps, keep_cold = c->cold_passes, keep_batch = c->interact_batch,
             keep_blocks = c->blocks_per_cu;
   float best_ms = INFINITY;
   ggh_launch_desc b{};
   for (int blocks : {1, 2})
-    for (int hot : {32, 40, 48})
+    for (int round : {12, 20, 32})
       for (int batch : {8, 16, 32}) {
         c->blocks_per_cu = blocks;
-        c->hot_threshold = hot;
-        c->resume_threshold = std::max(keep_res, hot);
+        c->round_steps = round;
         c->interact_batch = batch;
         HIP_TRY(hipMemcpyAsync(c->rng.p, saved.p, saved.n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
         if ((rc = ggh_run(c, n_trial))) break;
         if (c->last_ms < best_ms) {
           best_ms = c->last_ms;
-          b.hot_threshold = hot;
-          b.resume_threshold = c->resume_threshold;
+          b.round_steps = round;
+          b.cold_passes = keep_cold;
           b.interact_batch = batch;
           b.blocks_per_cu = blocks;
           b.general_kernel = c->force_general ? 1 : 0;
@@ -2348,14 +2480,14 @@ int ggh_autotune(ggh_context* c, uint64_t n_trial, ggh_launch_desc* best) {
   (void)hipStreamSynchronize(c->stream);
   saved.release();
   if (rc || e != hipSuccess) {
-    c->hot_threshold = keep_hot;
-    c->resume_threshold = keep_res;
+    c->round_steps = keep_round;
+    c->cold_passes = keep_cold;
     c->interact_batch = keep_batch;
     c->blocks_per_cu = keep_blocks;
     return rc ? rc : fail(-100 - (int)e, "autotune: restoring the random streams failed");
   }
-  c->hot_threshold = b.hot_threshold;
-  c->resume_threshold = b.resume_threshold;
+  c->round_steps = b.round_steps;
+  c->cold_passes = b.cold_passes;
   c->interact_batch = b.interact_batch;
   c->blocks_per_cu = b.blocks_per_cu;
   if (best) *best = b;

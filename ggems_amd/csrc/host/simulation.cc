@@ -845,6 +845,6 @@ void GGEMS::autotune(uint64_t n_trial) {
   upload_source(*sources.front());
   ggh_launch_desc best;
   engine_check(ggh_autotune(ctx, n_trial, &best), "autotune");
-  log(1, "GGEMS", "autotuned launch: hot threshold " + std::to_string(best.hot_threshold) + ", interaction batch " +
+  log(1, "GGEMS", "autotuned launch: round steps " + std::to_string(best.round_steps) + ", interaction batch " +
                       std::to_string(best.interact_batch) + ", blocks per CU " + std::to_string(best.blocks_per_cu));
 }

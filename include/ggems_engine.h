@@ -165,13 +165,13 @@ int ggh_get_stats(ggh_context* ctx, ggh_stats* out);
 /* launch shape of ggh_run (no counterpart in GGEMS; for tuning and tests).
  * Zero fields keep the defaults. */
 typedef struct {
-  int32_t hot_threshold;   /* a wave keeps stepping voxels while at least this many of its 64 lanes
-                              are inside a phantom (default 40) */
-  int32_t resume_threshold;/* the navigation phase ends once this many lanes step voxels again
-                              (default 64: once every lane is back inside a phantom or done) */
+  int32_t round_steps;     /* voxel steps a wave makes at least between two regroupings of its
+                              block's photons (1..1024, default 20) */
+  int32_t cold_passes;     /* navigation passes per round for the photons that are not inside a
+                              phantom (1..16, default 2) */
   int32_t interact_batch;  /* inside the voxel loop, resolve pending Compton/Rayleigh/photoelectric
                               interactions once this many lanes wait for one (default 16; 64 leaves
-                              them to the navigation phase) */
+                              them to the navigation passes) */
   int32_t blocks_per_cu;   /* cap on persistent blocks per compute unit (default: as many as fit) */
   int32_t general_kernel;  /* nonzero: the general kernel variant even where the specialised one applies */
 } ggh_launch_desc;
@@ -225,7 +225,7 @@ int ggh_replay_acquisition(ggh_context* ctx);
 int ggh_drr(ggh_context* ctx, int solid, int projection, float* image);
 
 /* Launch autotuner: times ggh_run(n_trial) over a grid of launch shapes
- * (blocks per compute unit, hot threshold, interaction batch) and keeps the
+ * (blocks per compute unit, round steps, interaction batch) and keeps the
  * fastest.  Call after ggh_seed and before any run: it restores the seeded
  * random streams and zeroes every tally and counter afterwards. */
 int ggh_autotune(ggh_context* ctx, uint64_t n_trial, ggh_launch_desc* best);
