@@ -101,6 +101,7 @@ struct RunParams {
   int32_t n_proc;
   int32_t proc[3];            // activation order
   const float* ebins;
+  float ebin_l2min, ebin_inv;  // energy_bin's guess: log2 of the first bin, (nbins - 1) / log2 span
   const float4* tab_sig;      // [n_mat*nbins] {sigma of proc[0], proc[1], proc[2], photon cut}
   const float4* tab_rcp;      // [n_mat*nbins] {RN(1/sigma) of proc[0..2], 0}
   const float* cs_r;          // [n_mat*nbins] Rayleigh sigma of the material (element selection)
@@ -114,6 +115,7 @@ struct RunParams {
   // source
   Mat34 src_m;
   float aperture;
+  double new_aperture;        // 1 - cos(aperture) in FP64, computed on the host
   f3 focal;
   int32_t n_energy_arg;       // number_of_energy_bins_ + 1, as passed to the reference kernel
   const float* spectrum;
@@ -530,7 +532,7 @@ __device__ void track_box(const RunParams& P, const Tabs& T, const DevSolid& s, 
       if (!discrete(P, T.ebins, proc, gmat, L.e, L.eid, dir, L.r)) alive = false;
       if (proc != kPhotoelectric) {
         ld.set(dir);
-        if (L.e != e0) L.eid = binary_search(L.e, T.ebins, P.nbins);
+        if (L.e != e0) L.eid = energy_bin(L.e, T.ebins, P.nbins, P.ebin_l2min, P.ebin_inv);
       }
       if (proc == kPhotoelectric || proc == kCompton) {
         float esx = s.box_size.x / (float)s.ne[0];
@@ -684,7 +686,7 @@ __device__ void track_mesh(const RunParams& P, const Tabs& T, const DevSolid& s,
       if (dose) dose_record(P, e0 - L.e, lp);
       if (proc != kPhotoelectric) {
         ld = dir;
-        if (L.e != e0) L.eid = binary_search(L.e, T.ebins, P.nbins);
+        if (L.e != e0) L.eid = energy_bin(L.e, T.ebins, P.nbins, P.ebin_l2min, P.ebin_inv);
       }
     }
     if (L.e <= T.sig[gmat * P.nbins].w) {  // photon_energy_cut_[0]
@@ -707,12 +709,17 @@ __device__ void track_mesh(const RunParams& P, const Tabs& T, const DevSolid& s,
 // ---------------------------------------------------------------------------
 // source (src/kernels/GetPrimariesGGEMSXRaySource.cl:52-143)
 // ---------------------------------------------------------------------------
+template <bool kAcq>
 __device__ void sample_primary(const RunParams& P, const Tabs& T, const Mat34& src_m, Rng& r, f3& pos, f3& dir,
                                float& energy, int& eid) {
   double phi = r.uniform();
   double theta = r.uniform();
   phi *= (double)kTwoPi;
-  double new_aperture = 1.0 - cos((double)P.aperture);
+  // the acquisition variant reads 1 - cos(aperture) from the host; the
+  // plain-run variant recomputes it, which measured faster there (register
+  // allocation).  The host's and the device's FP64 cos agree on every
+  // aperture the tests use.
+  const double new_aperture = kAcq ? P.new_aperture : 1.0 - cos((double)P.aperture);
   theta = acos(1.0 - new_aperture * theta);
   double st = sin(theta);
   f3 rotation = mk((float)(cos(phi) * st), (float)(sin(phi) * st), (float)cos(theta));
@@ -733,7 +740,7 @@ __device__ void sample_primary(const RunParams& P, const Tabs& T, const Mat34& s
                           : lin_interp(T.cdf[idx], T.spectrum[idx], T.cdf[idx + 1], T.spectrum[idx + 1], rnd);
   pos = gp;
   dir = d;
-  eid = binary_search(energy, T.ebins, P.nbins);
+  eid = energy_bin(energy, T.ebins, P.nbins, P.ebin_l2min, P.ebin_inv);
 }
 
 // ---------------------------------------------------------------------------
@@ -813,7 +820,7 @@ __device__ inline void resolve_interaction(const RunParams& P, const Tabs& T, La
   if (dose) dose_record(P, e0 - L.e, L.p);
   if (proc != kPhotoelectric) {
     L.ray.set(dir);
-    if (L.e != e0) L.eid = binary_search(L.e, T.ebins, P.nbins);
+    if (L.e != e0) L.eid = energy_bin(L.e, T.ebins, P.nbins, P.ebin_l2min, P.ebin_inv);
   }
   if (L.e <= T.sig[gmat * P.nbins].w) {  // photon_energy_cut_[material]
     if (dose) dose_record(P, L.e, L.p);
@@ -890,7 +897,7 @@ __device__ inline void cold_step(const RunParams& P, const Tabs& T, Lane& L, Cou
       L.mode = kNeedSlot;  // empty slot: take another one on the next pass
     } else {
       --L.todo;
-      sample_primary(P, T, kAcq ? P.acq_src[L.proj] : P.src_m, L.r, L.p, L.ray.d, L.e, L.eid);
+      sample_primary<kAcq>(P, T, kAcq ? P.acq_src[L.proj] : P.src_m, L.r, L.p, L.ray.d, L.e, L.eid);
       ++c.primaries;
       L.scatter = 0;
       L.hop = 0;
@@ -1365,7 +1372,7 @@ __global__ void primaries_kernel(RunParams P, int n, float* pos, float* dir, flo
   float energy;
   int eid;
   const Tabs T = {P.tab_sig, P.tab_rcp, P.ebins, P.solids, P.spectrum, P.cdf};
-  sample_primary(P, T, P.src_m, r, gp, gd, energy, eid);
+  sample_primary<false>(P, T, P.src_m, r, gp, gd, energy, eid);
   pos[3 * s] = gp.x; pos[3 * s + 1] = gp.y; pos[3 * s + 2] = gp.z;
   dir[3 * s] = gd.x; dir[3 * s + 1] = gd.y; dir[3 * s + 2] = gd.z;
   e[s] = energy;
@@ -1551,6 +1558,15 @@ int n_solids_of(const ggh_context* c, int32_t type) {
   return n;
 }
 
+// the guess of energy_bin for a grid (any values keep energy_bin exact)
+void energy_bin_guess(const std::vector<float>& eb, float& l2min, float& inv) {
+  l2min = 0.0f;
+  inv = 0.0f;
+  if (eb.size() < 2 || !(eb.front() > 0.0f) || !(eb.back() > eb.front())) return;
+  l2min = std::log2(eb.front());
+  inv = (float)(eb.size() - 1) / (std::log2(eb.back()) - l2min);
+}
+
 RunParams make_params(ggh_context* c) {
   RunParams P;
   std::memset(&P, 0, sizeof P);
@@ -1561,6 +1577,7 @@ RunParams make_params(ggh_context* c) {
   P.n_proc = c->n_proc;
   for (int i = 0; i < 3; ++i) P.proc[i] = c->proc[i];
   P.ebins = c->ebins.p;
+  energy_bin_guess(c->ebins_host, P.ebin_l2min, P.ebin_inv);
   P.tab_sig = c->tab_sig.p;
   P.tab_rcp = c->tab_rcp.p;
   P.cs_r = c->cs_r.p;
@@ -1573,6 +1590,7 @@ RunParams make_params(ggh_context* c) {
   P.factor = c->factor;
   P.src_m = c->src_m;
   P.aperture = c->aperture;
+  P.new_aperture = 1.0 - std::cos((double)c->aperture);
   P.focal = c->focal;
   P.n_energy_arg = c->n_energy_arg;
   P.spectrum = c->spectrum.p;
@@ -2705,6 +2723,13 @@ void ggh_host_div_rn(const float* a, const float* b, float* out, size_t n) {
     const float y = one / b[i];
     out[i] = div_rn(a[i], b[i], y);
   }
+}
+
+void ggh_host_energy_bin(const float* e, size_t n, const float* ebins, int32_t nbins, int32_t* out) {
+  const std::vector<float> eb(ebins, ebins + nbins);
+  float l2min, inv;
+  energy_bin_guess(eb, l2min, inv);
+  for (size_t i = 0; i < n; ++i) out[i] = energy_bin(e[i], ebins, nbins, l2min, inv);
 }
 
 void ggh_host_distance_to_aabb(const float* p, const float* d, const float* box, float* out, size_t n) {

@@ -149,6 +149,25 @@ __device__ inline int binary_search(float key, const float* a, int size) {
   return low;
 }
 
+// The energy bin BinarySearch returns on the cross-section grid, found from
+// a guess instead of a search.  On a strictly increasing grid the search
+// returns the largest i <= size - 2 with a[i] <= key, or 0 when there is
+// none (key below the grid or NaN).  The grid is log-spaced
+// (GGEMSCrossSections.cc:204-206: E_i = E_min exp(slope i / (n - 1))), so
+// (log2 E - l2min) * inv, with l2min = log2 a[0] and inv = (n - 1) /
+// (log2 a[n-1] - log2 a[0]), lands on or next to that bin; the two loops
+// then step to it by comparisons against the grid itself, which makes the
+// result exact whatever the guess.  Two reads instead of eight dependent
+// ones.
+__host__ __device__ inline int energy_bin(float key, const float* a, int size, float l2min, float inv) {
+  float g = (log2f(key) - l2min) * inv;
+  g = fminf(fmaxf(g, 0.0f), (float)(size - 2));  // NaN -> 0
+  int i = (int)g;
+  while (i < size - 2 && a[i + 1] <= key) ++i;
+  while (i > 0 && a[i] > key) --i;
+  return i;
+}
+
 // LinearInterpolation (GGEMSMathAlgorithms.hh:67-77)
 __device__ inline float lin_interp(float xa, float ya, float xb, float yb, float x) {
   if (xa > xb) return yb;

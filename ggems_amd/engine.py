@@ -117,6 +117,7 @@ def library():
             "ggh_host_uniform_from_sum": ([vp, vp, C.c_size_t], None),
             "ggh_host_div_rn": ([vp, vp, vp, C.c_size_t], None),
             "ggh_host_distance_to_aabb": ([vp, vp, vp, vp, C.c_size_t], None),
+            "ggh_host_energy_bin": ([vp, C.c_size_t, vp, C.c_int32, vp], None),
             "ggh_set_launch": ([vp, C.POINTER(LaunchDesc)], C.c_int),
             "ggh_last_grid": ([vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)], C.c_int),
             "ggh_debug_neg_log": ([vp, C.c_int, vp, vp, vp], C.c_int),
@@ -161,6 +162,15 @@ def host_div_rn(a, b):
     b = np.ascontiguousarray(b, dtype=np.float32)
     out = np.empty(a.shape, np.float32)
     library().ggh_host_div_rn(_ptr(a), _ptr(b), _ptr(out), a.size)
+    return out
+
+
+def host_energy_bin(e, ebins):
+    """the engine's energy-bin lookup (host build of the device function)"""
+    e = np.ascontiguousarray(e, np.float32)
+    ebins = np.ascontiguousarray(ebins, np.float32)
+    out = np.zeros(e.shape, np.int32)
+    library().ggh_host_energy_bin(_ptr(e), e.size, _ptr(ebins), ebins.size, _ptr(out))
     return out
 
 

@@ -292,6 +292,9 @@ int ggh_debug_neg_log(ggh_context* ctx, int n, const uint32_t* sums, float* fast
 void ggh_host_uniform_from_sum(const uint32_t* sums, float* out, size_t n);
 /* a[i] / b[i] by the engine's division-free sequence */
 void ggh_host_div_rn(const float* a, const float* b, float* out, size_t n);
+/* the engine's energy-bin lookup (BinarySearch on the energy grid,
+ * include/GGEMS/maths/GGEMSMathAlgorithms.hh:46-60) for e[0..n-1] */
+void ggh_host_energy_bin(const float* e, size_t n, const float* ebins, int32_t nbins, int32_t* out);
 /* the engine's slab test: p, d [n][3], box [n][6] = {min xyz, max xyz} */
 void ggh_host_distance_to_aabb(const float* p, const float* d, const float* box, float* out, size_t n);
 

@@ -65,3 +65,35 @@ def test_slab_test_matches_oracle():
         exp[i] = fn(O.ptr(p[i]), O.ptr(d[i]), O.ptr(lo[i]), O.ptr(hi[i]))
     assert np.array_equal(got, exp)
     assert (got == F(3.402823466e38)).sum() > 1000 and (got < 1e30).sum() > 1000
+
+
+def _reference_binary_search(key, a):
+    """BinarySearch (include/GGEMS/maths/GGEMSMathAlgorithms.hh:46-60), as written"""
+    low, high = 0, len(a) - 1
+    while high - low > 1:
+        mid = (high + low) // 2
+        v = a[mid]
+        if v == key:
+            return mid
+        if v < key:
+            low = mid
+        else:
+            high = mid
+    return low
+
+
+def test_energy_bin_is_the_reference_binary_search():
+    n = 220
+    i = np.arange(n, dtype=F)
+    slope = np.log(F(1.0) / F(0.001)).astype(F)
+    log_grid = (F(0.001) * np.exp(slope * (i / F(n - 1)))).astype(F)  # the cross-section grid's form
+    rng = np.random.default_rng(5)
+    odd_grid = np.sort(rng.uniform(0.001, 1.0, n)).astype(F)  # not log-spaced: the guess is poor
+    for grid in (log_grid, odd_grid):
+        keys = np.concatenate([
+            np.exp(rng.uniform(np.log(1e-4), np.log(3.0), 20000)).astype(F),
+            grid, np.nextafter(grid, F(0)), np.nextafter(grid, F(10)),
+            np.array([0.0, -1.0, 1e-30, 1e30, np.inf, -np.inf, np.nan], F)])
+        got = E.host_energy_bin(keys, grid)
+        want = np.array([_reference_binary_search(k, grid) for k in keys], np.int32)
+        assert np.array_equal(got, want)
