@@ -129,6 +129,33 @@ def test_cfg1_run_matches_oracle(cfg1, general, interact_batch):
     eng.close()
 
 
+def test_two_phantoms_match_oracle():
+    """Two voxel phantoms side by side (one of them rotated), so that one wave
+    holds photons in different voxel solids: the general kernel's regrouping
+    and its per-lane phantom geometry against the oracle, history by history."""
+    sc = Scene()
+    sc.add_phantom(cylinder_phantom(40), ["Water", "RibBone"], (1.0, 1.0, 1.0), position=(0.0, -45.0, 0.0))
+    sc.add_phantom(cylinder_phantom(32), ["Water", "RibBone"], (1.25, 1.25, 1.25), position=(0.0, 45.0, 0.0),
+                   rotation=(0.0, 0.0, 30.0), dosimetry={})
+    sc.add_ct_flat("GOS", (1, 1), (128, 128, 1), (1.5, 1.5, 1.0), 600.0, 1000.0, 10e-3)
+    sc.set_source((-600.0, 0.0, 0.0), 8.0, [0.06], mono=True)
+    sc.build()
+    n = 200_000
+    ref = sc.run_oracle(n, 31, threads=0)
+    eng = sc.make_engine()
+    got = sc.run_engine(n, 31, eng)
+    assert eng.last_grid()[2] == 1
+    rs, gs = ref["stats"], got["stats"]
+    assert gs["primaries"] == rs["primaries"] == n
+    assert rs["voxel_steps"] > 2 * n and ref["edep"].sum() > 0  # both phantoms are crossed
+    for k in ("voxel_steps", "interactions", "hops"):
+        assert abs(gs[k] - rs[k]) <= 0.01 * rs[k] + 10, (k, gs[k], rs[k])
+    h_ref, h_got = ref["hist"][0], got["hist"][0]
+    assert np.abs(h_ref.astype(np.int64) - h_got).sum() <= 0.02 * h_ref.sum()
+    assert abs(got["edep"].sum() - ref["edep"].sum()) <= 0.01 * ref["edep"].sum()
+    eng.close()
+
+
 def _chi2_two_sample(a, b):
     """chi-square of two independent Poisson samples of one distribution, p-value."""
     from scipy.stats import chi2
