@@ -60,23 +60,42 @@ __device__ inline bf16x8 load_a(const __bf16* x, int img, int y, int xx, int k0,
   return *reinterpret_cast<const bf16x8*>(x + (((size_t)img * H + yy) * W + xs) * Cin + ci);
 }
 
-// The block's four waves share one 64-channel tile and take four
-// consecutive 32-pixel tiles; each K step's weight chunk (64 channels x 32 k,
-// 4 KB) is staged once per block in LDS, double-buffered, instead of being
-// fetched from L2 by every wave.
+// The blocks go to the eight XCDs in turn (block b to XCD b % 8), and each
+// XCD has its own L2.  xcd_tile() numbers the tiles so that each XCD takes a
+// contiguous run of them: neighbouring pixel tiles, which read the same
+// image rows through the 3x3 stencil, then share one L2.  Blocks past the
+// last tile return at once.
+constexpr int kXcds = 8;
+
+__device__ inline int xcd_tile(int n_tiles) {
+  const int per = (int)((gridDim.x + kXcds - 1) / kXcds);
+  return (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
+}
+
+// The block's four waves share one tile of 16 NT output channels and take
+// four consecutive 32-pixel tiles; each K step's weight chunk (16 NT
+// channels x 32 k) is staged once per block in LDS, double-buffered, instead
+// of being fetched from L2 by every wave.  NT is 4 for 64 or more output
+// channels and 1 or 2 for the narrow layers, which would otherwise multiply
+// zero weights.
 constexpr int kBRow = 32 + 8;  // bf16 per staged weight row (padded against bank conflicts)
 
+template <int NT>
 __global__ __launch_bounds__(64 * kWaves) void conv_fwd_kernel(const __bf16* __restrict__ x,
                                                                const __bf16* __restrict__ w,
                                                                const float* __restrict__ bias,
                                                                __bf16* __restrict__ y, int N, int H, int W, int Cin,
                                                                int Cout, int relu) {
-  __shared__ __bf16 sB[2][kFwdCh * kBRow];
+  constexpr int kCh = 16 * NT;
+  __shared__ __bf16 sB[2][kCh * kBRow];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int pix_tiles = (N * H * W) / kFwdPix;
-  const int ch_tiles = (Cout + kFwdCh - 1) / kFwdCh;
-  const int ct = blockIdx.x % ch_tiles;
-  const int pt = (blockIdx.x / ch_tiles) * kWaves + wave;
+  const int ch_tiles = (Cout + kCh - 1) / kCh;
+  const int block_tiles = ((pix_tiles + kWaves - 1) / kWaves) * ch_tiles;
+  const int bt = xcd_tile(block_tiles);
+  if (bt >= block_tiles) return;  // the whole block: no barrier is skipped
+  const int ct = bt % ch_tiles;
+  const int pt = (bt / ch_tiles) * kWaves + wave;
   const bool active = pt < pix_tiles;  // every wave stages and synchronises; only active ones compute
   const int p0 = (active ? pt : 0) * kFwdPix;
   // each run of 16 pixels lies in one image row (W is a multiple of 16)
@@ -87,36 +106,39 @@ __global__ __launch_bounds__(64 * kWaves) void conv_fwd_kernel(const __bf16* __r
     yrow[i] = rem / W;
     x0[i] = rem - yrow[i] * W;
   }
-  const int c0 = ct * kFwdCh;
+  const int c0 = ct * kCh;
   const int K = 9 * Cin;
   const int r = lane & 15, g = lane >> 4;  // fragment row / k group
-  // staging: thread t copies 8 k of channel c0 + t / 4
+  // staging: thread t < 4 kCh copies 8 k of channel c0 + t / 4
   const int s_co = threadIdx.x >> 2, s_kq = (threadIdx.x & 3) * 8;
+  const bool stager = s_co < kCh;
   auto stage = [&](int buf, int k0) {
+    if (!stager) return;
     const int co = c0 + s_co, k = k0 + s_kq;
     bf16x8 v = (co < Cout && k < K) ? *reinterpret_cast<const bf16x8*>(w + (size_t)co * K + k) : zero8();
     *reinterpret_cast<bf16x8*>(&sB[buf][s_co * kBRow + s_kq]) = v;
   };
-  f32x4 acc[2][4];
+  f32x4 acc[2][NT];
   for (int i = 0; i < 2; ++i)
-    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < NT; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
   stage(0, 0);
   __syncthreads();
   int buf = 0;
   for (int k0 = 0; k0 < K; k0 += 32) {
     if (k0 + 32 < K) stage(buf ^ 1, k0 + 32);  // the next chunk, while this one is used
     const int kk = k0 + 8 * g;
-    bf16x8 a[2], b[4];
+    bf16x8 a[2], b[NT];
     for (int i = 0; i < 2; ++i) a[i] = load_a(x, img[i], yrow[i], x0[i] + r, kk, K, Cin, H, W);
-    for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const bf16x8*>(&sB[buf][(16 * j + r) * kBRow + 8 * g]);
+    for (int j = 0; j < NT; ++j) b[j] = *reinterpret_cast<const bf16x8*>(&sB[buf][(16 * j + r) * kBRow + 8 * g]);
     for (int i = 0; i < 2; ++i)
-      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+      for (int j = 0; j < NT; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     __syncthreads();
     buf ^= 1;
   }
   if (!active) return;
   // C fragment: column r (output channel), rows 4 g .. 4 g + 3 (pixels)
-  for (int j = 0; j < 4; ++j) {
+  for (int j = 0; j < NT; ++j) {
     const int co = c0 + 16 * j + r;
     if (co >= Cout) continue;
     const float bv = bias ? bias[co] : 0.0f;
@@ -144,7 +166,10 @@ __global__ __launch_bounds__(64 * kWaves) void conv_fwd_splitk_kernel(const __bf
   __shared__ float sP[kWaves][32][64 + 1];  // partial tiles: [wave][element][lane]
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int ch_tiles = (Cout + kFwdCh - 1) / kFwdCh;
-  const int ct = blockIdx.x % ch_tiles, pt = blockIdx.x / ch_tiles;
+  const int n_tiles = ((N * H * W) / kFwdPix) * ch_tiles;
+  const int bt = xcd_tile(n_tiles);
+  if (bt >= n_tiles) return;  // the whole block: no barrier is skipped
+  const int ct = bt % ch_tiles, pt = bt / ch_tiles;
   const int p0 = pt * kFwdPix;
   int img[2], yrow[2], x0[2];
   for (int i = 0; i < 2; ++i) {
@@ -283,14 +308,25 @@ int ggh_conv3x3_forward(const void* x, const void* w, const float* bias, void* y
   const int pix_tiles = n * h * wd / kFwdPix, ch_tiles = (cout + kFwdCh - 1) / kFwdCh;
   // deep layers with few output tiles: split K over the block's waves
   const bool split = 9 * cin >= 8 * 32 * kWaves && pix_tiles * ch_tiles < 1024;
-  if (split)
-    hipLaunchKernelGGL(conv_fwd_splitk_kernel, dim3(pix_tiles * ch_tiles), dim3(64 * kWaves), 0,
-                       static_cast<hipStream_t>(stream), static_cast<const __bf16*>(x),
-                       static_cast<const __bf16*>(w), bias, static_cast<__bf16*>(y), n, h, wd, cin, cout, relu);
-  else
-    hipLaunchKernelGGL(conv_fwd_kernel, dim3(((pix_tiles + kWaves - 1) / kWaves) * ch_tiles), dim3(64 * kWaves), 0,
-                       static_cast<hipStream_t>(stream), static_cast<const __bf16*>(x),
-                       static_cast<const __bf16*>(w), bias, static_cast<__bf16*>(y), n, h, wd, cin, cout, relu);
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  const __bf16 *xb = static_cast<const __bf16*>(x), *wb = static_cast<const __bf16*>(w);
+  __bf16* yb = static_cast<__bf16*>(y);
+  if (split) {
+    const dim3 grid((unsigned)((pix_tiles * ch_tiles + kXcds - 1) / kXcds * kXcds));
+    hipLaunchKernelGGL(conv_fwd_splitk_kernel, grid, dim3(64 * kWaves), 0, st, xb, wb, bias, yb, n, h, wd, cin, cout,
+                       relu);
+  } else {
+    // output channels per wave: 64, or 16 / 32 for the narrow layers
+    const int nt = cout >= 64 ? 4 : (cout >= 32 ? 2 : 1);
+    const int block_tiles = ((pix_tiles + kWaves - 1) / kWaves) * ((cout + 16 * nt - 1) / (16 * nt));
+    const dim3 grid((unsigned)((block_tiles + kXcds - 1) / kXcds * kXcds));
+    if (nt == 4)
+      hipLaunchKernelGGL(conv_fwd_kernel<4>, grid, dim3(64 * kWaves), 0, st, xb, wb, bias, yb, n, h, wd, cin, cout, relu);
+    else if (nt == 2)
+      hipLaunchKernelGGL(conv_fwd_kernel<2>, grid, dim3(64 * kWaves), 0, st, xb, wb, bias, yb, n, h, wd, cin, cout, relu);
+    else
+      hipLaunchKernelGGL(conv_fwd_kernel<1>, grid, dim3(64 * kWaves), 0, st, xb, wb, bias, yb, n, h, wd, cin, cout, relu);
+  }
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return ufail(-100 - (int)e, std::string("conv forward: ") + hipGetErrorString(e));
   return 0;
