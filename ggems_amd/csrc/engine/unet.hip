@@ -230,8 +230,10 @@ __global__ __launch_bounds__(64 * kWaves) void conv_wgrad_kernel(const __bf16* _
                                                                  const __bf16* __restrict__ dy,
                                                                  float* __restrict__ dw, int N, int H, int W,
                                                                  int Cin, int Cout, int pix_per_split) {
-  __shared__ __bf16 s_dy[kWgMaxM][kWgPix + 8];  // [co][pixel], padded rows
-  __shared__ __bf16 s_x[kWgN][kWgPix + 8];      // [n][pixel]
+  // double-buffered: step k + 1's global loads are in flight while step k's
+  // MFMAs run, and land in the other buffer
+  __shared__ __bf16 s_dy[2][kWgMaxM][kWgPix + 8];  // [buf][co][pixel], padded rows
+  __shared__ __bf16 s_x[2][kWgN][kWgPix + 8];      // [buf][n][pixel]
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int K9 = 9 * Cin;
   const int n0 = blockIdx.x * kWgN;
@@ -240,42 +242,71 @@ __global__ __launch_bounds__(64 * kWaves) void conv_wgrad_kernel(const __bf16* _
   const int pend = min(P, pbeg + pix_per_split);
   const int r = lane & 15, g = lane >> 4;
   const int m_tiles = Cout / 16;
-  // wave w owns column tile w (16 of the block's 64 columns) and every row tile
-  f32x4 acc[kWgMaxM / 16];
-  for (int i = 0; i < kWgMaxM / 16; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int pc = pbeg; pc < pend; pc += kWgPix) {
-    // stage dY (32 pixels x Cout) transposed: thread t loads 8 channels of one pixel
-    for (int t = threadIdx.x; t < kWgPix * (Cout / 8); t += 64 * kWaves) {
-      const int p = t / (Cout / 8), c8 = (t - p * (Cout / 8)) * 8;
-      const int px = pc + p;
-      bf16x8 v = px < pend ? *reinterpret_cast<const bf16x8*>(dy + (size_t)px * Cout + c8) : zero8();
-      for (int e = 0; e < 8; ++e) s_dy[c8 + e][p] = v[e];
+  // per thread: up to two 8-channel pieces of dY (Cout <= 128: 32 x 16 pieces)
+  // and one 8-column piece of the shifted input (32 x 8 pieces)
+  constexpr int kDyPieces = kWgPix * (kWgMaxM / 8) / (64 * kWaves);
+  const int n_dy = kWgPix * (Cout / 8);
+  bf16x8 vdy[kDyPieces], vx;
+  auto load = [&](int pc) {
+    for (int q = 0; q < kDyPieces; ++q) {
+      const int t = threadIdx.x + q * 64 * kWaves;
+      vdy[q] = zero8();
+      if (t < n_dy) {
+        const int p = t / (Cout / 8), c8 = (t - p * (Cout / 8)) * 8;
+        const int px = pc + p;
+        if (px < pend) vdy[q] = *reinterpret_cast<const bf16x8*>(dy + (size_t)px * Cout + c8);
+      }
     }
-    // stage the shifted input for this block's 64 columns (8 consecutive ci share a tap)
-    for (int t = threadIdx.x; t < kWgPix * (kWgN / 8); t += 64 * kWaves) {
+    {
+      const int t = threadIdx.x;  // kWgPix * kWgN / 8 == 64 * kWaves pieces
       const int p = t / (kWgN / 8), n8 = (t - p * (kWgN / 8)) * 8;
       const int n = n0 + n8, px = pc + p;
-      bf16x8 v = zero8();
+      vx = zero8();
       if (px < pend && n < K9) {
         const int tap = n / Cin, ci = n - tap * Cin;
         const int img = px / (H * W), rem = px - img * H * W, yy = rem / W + tap / 3 - 1, xs = rem % W + tap % 3 - 1;
         if (yy >= 0 && yy < H && xs >= 0 && xs < W)
-          v = *reinterpret_cast<const bf16x8*>(x + (((size_t)img * H + yy) * W + xs) * Cin + ci);
+          vx = *reinterpret_cast<const bf16x8*>(x + (((size_t)img * H + yy) * W + xs) * Cin + ci);
       }
-      for (int e = 0; e < 8; ++e) s_x[n8 + e][p] = v[e];
     }
-    __syncthreads();
+  };
+  auto store = [&](int buf) {  // transposed: pixels contiguous
+    for (int q = 0; q < kDyPieces; ++q) {
+      const int t = threadIdx.x + q * 64 * kWaves;
+      if (t < n_dy) {
+        const int p = t / (Cout / 8), c8 = (t - p * (Cout / 8)) * 8;
+        for (int e = 0; e < 8; ++e) s_dy[buf][c8 + e][p] = vdy[q][e];
+      }
+    }
+    const int t = threadIdx.x;
+    const int p = t / (kWgN / 8), n8 = (t - p * (kWgN / 8)) * 8;
+    for (int e = 0; e < 8; ++e) s_x[buf][n8 + e][p] = vx[e];
+  };
+  // wave w owns column tile w (16 of the block's 64 columns) and every row tile
+  f32x4 acc[kWgMaxM / 16];
+  for (int i = 0; i < kWgMaxM / 16; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+  int buf = 0;
+  if (pbeg < pend) {
+    load(pbeg);
+    store(0);
+  }
+  __syncthreads();
+  for (int pc = pbeg; pc < pend; pc += kWgPix) {
+    const bool more = pc + kWgPix < pend;
+    if (more) load(pc + kWgPix);
     // B fragment: column (n) r of the wave's tile, pixels 8 g .. 8 g + 7
     bf16x8 b;
-    for (int e = 0; e < 8; ++e) b[e] = s_x[16 * wave + r][8 * g + e];
+    for (int e = 0; e < 8; ++e) b[e] = s_x[buf][16 * wave + r][8 * g + e];
 #pragma unroll
     for (int i = 0; i < kWgMaxM / 16; ++i) {
       if (i >= m_tiles) break;
       bf16x8 a;
-      for (int e = 0; e < 8; ++e) a[e] = s_dy[16 * i + r][8 * g + e];
+      for (int e = 0; e < 8; ++e) a[e] = s_dy[buf][16 * i + r][8 * g + e];
       acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i], 0, 0, 0);
     }
+    if (more) store(buf ^ 1);  // the other buffer: nobody reads it during this step
     __syncthreads();
+    buf ^= 1;
   }
   // C fragment: column n = n0 + 16 wave + r, rows co = 16 i + 4 g + e
   const int n = n0 + 16 * wave + r;

@@ -78,7 +78,11 @@ def conv3x3_weight_grad(x, dy, splits=None):
     cout = dy.shape[3]
     dw = torch.zeros((cout, 9 * cin), dtype=torch.float32, device=x.device)
     if splits is None:
-        splits = max(1, min(1024, (n * h * wd) // 1024))
+        # pixel splits: about 1,024 pixels each, and at least 1,024 blocks in
+        # all (cols x splits) so that the small deep layers fill the chip
+        cols = (9 * cin + 63) // 64
+        p = n * h * wd
+        splits = max(1, min(p // 32, max(p // 1024, -(-1024 // cols))))
     _check(_lib().ggh_conv3x3_weight_grad(x.data_ptr(), dy.data_ptr(), dw.data_ptr(), n, h, wd, cin, cout,
                                           int(splits), _stream(x)))
     return dw
