@@ -1455,6 +1455,7 @@ Mat34 to_mat34(const float* m16) {
 struct ggh_context {
   int device = 0;
   int n_cu = 0;
+  int max_lds = 0;  // bytes of LDS one block may allocate
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float last_ms = 0.0f;
@@ -1711,14 +1712,16 @@ struct Launch {
 // block regrouping whenever the scene has a voxel solid to step through
 int pick_launch(ggh_context* c, uint64_t slots, bool acq, Launch& L) {
   const bool single = n_solids_of(c, kVoxel) == 1 && n_solids_of(c, kMesh) == 0;
-  const size_t lds = tables_lds_bytes(c);
-  const bool use_lds = lds <= (size_t)kLdsBudget;
-  const bool special = c->n_proc == 3 && single && use_lds && !c->ptrack.p && !c->force_general;
   const bool regroup = n_solids_of(c, kVoxel) > 0;
   // dynamic LDS: the staged tables (when they fit), then the lane-state
   // exchange area of the block's regrouping
-  L.xchg_offset = use_lds ? (uint32_t)((lds + 15) & ~(size_t)15) : 0u;
-  L.shmem = L.xchg_offset + (regroup ? kXchgBytes : 0);
+  const size_t xchg = regroup ? kXchgBytes : 0;
+  const size_t lds = (tables_lds_bytes(c) + 15) & ~(size_t)15;
+  const bool use_lds = lds <= (size_t)kLdsBudget && lds + xchg <= (size_t)c->max_lds;
+  if (xchg > (size_t)c->max_lds) return fail(-3, "the block's lane-state exchange area does not fit in LDS");
+  const bool special = c->n_proc == 3 && single && use_lds && !c->ptrack.p && !c->force_general;
+  L.xchg_offset = use_lds ? (uint32_t)lds : 0u;
+  L.shmem = L.xchg_offset + xchg;
   if (special)
     L.fn = acq ? transport_kernel<3, true, true, false, true, true>
                : transport_kernel<3, true, true, false, false, true>;
@@ -1798,6 +1801,7 @@ int ggh_context_create(int device, ggh_context** out) {
   if (!c) return fail(-3, "out of host memory");
   c->device = device;
   hipError_t e = hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&c->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
