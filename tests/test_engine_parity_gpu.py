@@ -129,6 +129,28 @@ def test_cfg1_run_matches_oracle(cfg1, general, interact_batch):
     eng.close()
 
 
+@pytest.mark.parametrize("n", [1, 3, 1_048_576 + 4_097])
+def test_batch_layout_edge_cases_match_oracle(cfg1, n):
+    """Runs of 1 and 3 primaries, and one just over a batch: GGEMS splits it
+    into two batches of 526,337 and 526,336 photons
+    (src/sources/GGEMSSource.cc:403-425), so slot 526,336 carries one photon
+    and every lower slot two."""
+    ref = cfg1.run_oracle(n, 4242, threads=0)
+    eng = cfg1.make_engine()
+    got = cfg1.run_engine(n, 4242, eng)
+    rs, gs = ref["stats"], got["stats"]
+    assert gs["primaries"] == rs["primaries"] == n
+    h_ref, h_got = ref["hist"][0], got["hist"][0]
+    if n < 10:
+        # a few histories: identical unless an ulp of a transcendental flips one
+        assert gs["voxel_steps"] == rs["voxel_steps"] and np.array_equal(h_ref, h_got)
+    else:
+        for k in ("voxel_steps", "interactions", "hops"):
+            assert abs(gs[k] - rs[k]) <= 0.01 * rs[k] + 10, (k, gs[k], rs[k])
+        assert np.abs(h_ref.astype(np.int64) - h_got).sum() <= 0.02 * h_ref.sum()
+    eng.close()
+
+
 def test_two_phantoms_match_oracle():
     """Two voxel phantoms side by side (one of them rotated), so that one wave
     holds photons in different voxel solids: the general kernel's regrouping
