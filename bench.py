@@ -274,9 +274,9 @@ def main():
     ap.add_argument("--n-low", type=int, default=1_000_000, help="surrogate: primaries of the input projection")
     ap.add_argument("--n-high", type=int, default=100_000_000, help="surrogate: primaries of the target")
     ap.add_argument("--n-eval", type=int, default=1_000_000_000, help="surrogate: primaries of held-out targets")
-    ap.add_argument("--train-views", type=int, default=48)
+    ap.add_argument("--train-views", type=int, default=192)
     ap.add_argument("--test-views", type=int, default=12)
-    ap.add_argument("--train-steps", type=int, default=1500)
+    ap.add_argument("--train-steps", type=int, default=8000)
     ap.add_argument("--angles", type=int, default=360, help="acquisition: projections per phantom")
     ap.add_argument("--budget-scale", type=float, default=0.1,
                     help="acquisition: budgets uniform in [1e6, 3e7] times this")

@@ -321,6 +321,55 @@ __global__ __launch_bounds__(64 * kWaves) void conv_wgrad_kernel(const __bf16* _
   }
 }
 
+// The backward of a layer's ReLU and bias in one pass: dy_out = dy where
+// the layer's output y > 0 (everywhere without ReLU), as BF16, and
+// db[c] += sum over pixels of dy_out[., c] in FP32.  A thread takes 8
+// channels of a pixel; the grid's thread count is a multiple of C / 8, so a
+// thread keeps the same 8 channels over its whole grid-stride loop, sums
+// them in registers, then adds them into the block's LDS sums, which one
+// global atomic per channel and block finishes.
+constexpr int kMaskBlock = 256;
+
+template <bool kF32>
+__global__ __launch_bounds__(kMaskBlock) void relu_bias_backward_kernel(const void* __restrict__ dy_in,
+                                                                        const __bf16* __restrict__ y, int relu,
+                                                                        __bf16* __restrict__ dy_out,
+                                                                        float* __restrict__ db, long long pieces,
+                                                                        int C) {
+  __shared__ float s_db[kWgMaxM];
+  for (int i = threadIdx.x; i < C; i += kMaskBlock) s_db[i] = 0.0f;
+  __syncthreads();
+  const int groups = C / 8;
+  const long long stride = (long long)gridDim.x * kMaskBlock;
+  long long q = (long long)blockIdx.x * kMaskBlock + threadIdx.x;
+  const int c8 = (int)(q % groups) * 8;  // the same for every q of this thread: stride % groups == 0
+  float acc[8];
+  for (int e = 0; e < 8; ++e) acc[e] = 0.0f;
+  for (; q < pieces; q += stride) {
+    float v[8];
+    if (kF32) {
+      const float4* src = reinterpret_cast<const float4*>(static_cast<const float*>(dy_in) + q * 8);
+      const float4 a = src[0], b = src[1];
+      v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else {
+      const bf16x8 d = reinterpret_cast<const bf16x8*>(dy_in)[q];
+      for (int e = 0; e < 8; ++e) v[e] = (float)d[e];
+    }
+    bf16x8 o;
+    if (relu) {
+      const bf16x8 yy = reinterpret_cast<const bf16x8*>(y)[q];
+      for (int e = 0; e < 8; ++e) o[e] = (float)yy[e] > 0.0f ? (__bf16)v[e] : (__bf16)0.0f;
+    } else {
+      for (int e = 0; e < 8; ++e) o[e] = (__bf16)v[e];
+    }
+    reinterpret_cast<bf16x8*>(dy_out)[q] = o;
+    for (int e = 0; e < 8; ++e) acc[e] += (float)o[e];
+  }
+  for (int e = 0; e < 8; ++e) atomicAdd(&s_db[c8 + e], acc[e]);
+  __syncthreads();
+  for (int i = threadIdx.x; i < C; i += kMaskBlock) unsafeAtomicAdd(&db[i], s_db[i]);
+}
+
 bool shapes_ok(int N, int H, int W, int Cin, int Cout) {
   return N > 0 && H > 0 && W > 0 && W % 16 == 0 && ((long long)N * H * W) % kFwdPix == 0 && Cin > 0 &&
          Cin % 8 == 0 && Cout > 0 && Cout % 16 == 0 && (long long)N * H * W * (Cin > Cout ? Cin : Cout) < (1ll << 31);
@@ -360,6 +409,27 @@ int ggh_conv3x3_forward(const void* x, const void* w, const float* bias, void* y
   }
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return ufail(-100 - (int)e, std::string("conv forward: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int ggh_relu_bias_backward(const void* dy, int32_t dy_is_f32, const void* y, int32_t relu, void* dy_out, float* db,
+                           int64_t n_pixels, int32_t c, void* stream) {
+  if (!dy || !dy_out || !db || (relu && !y)) return ufail(-1, "null pointer");
+  // C / 8 must divide the block size (a thread keeps its 8 channels)
+  if (n_pixels <= 0 || (c != 8 && c != 16 && c != 32 && c != 64 && c != 128))
+    return ufail(-2, "unsupported shape (C one of 8, 16, 32, 64, 128)");
+  const long long pieces = (long long)n_pixels * (c / 8);
+  const long long want = (pieces + kMaskBlock - 1) / kMaskBlock;
+  const unsigned blocks = (unsigned)(want < 1024 ? want : 1024);
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dy_is_f32)
+    hipLaunchKernelGGL(relu_bias_backward_kernel<true>, dim3(blocks), dim3(kMaskBlock), 0, st, dy,
+                       static_cast<const __bf16*>(y), relu, static_cast<__bf16*>(dy_out), db, pieces, c);
+  else
+    hipLaunchKernelGGL(relu_bias_backward_kernel<false>, dim3(blocks), dim3(kMaskBlock), 0, st, dy,
+                       static_cast<const __bf16*>(y), relu, static_cast<__bf16*>(dy_out), db, pieces, c);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return ufail(-100 - (int)e, std::string("ReLU / bias backward: ") + hipGetErrorString(e));
   return 0;
 }
 

@@ -32,6 +32,8 @@ _SIGS = {
     "ggh_conv3x3_forward": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p] + [C.c_int32] * 6 + [C.c_void_p],
                             C.c_int),
     "ggh_conv3x3_weight_grad": ([C.c_void_p, C.c_void_p, C.c_void_p] + [C.c_int32] * 6 + [C.c_void_p], C.c_int),
+    "ggh_relu_bias_backward": ([C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64,
+                                C.c_int32, C.c_void_p], C.c_int),
 }
 
 
@@ -88,6 +90,22 @@ def conv3x3_weight_grad(x, dy, splits=None):
     return dw
 
 
+def relu_bias_backward(dy, y, relu):
+    """(dy masked where the layer's output y <= 0 when relu, as BF16; the bias
+    gradient, the sum of that over the pixels in FP32) in one kernel"""
+    if dy.dtype not in (torch.bfloat16, torch.float32):
+        dy = dy.float()
+    dy = dy.contiguous()
+    c = dy.shape[-1]
+    yc = y.contiguous() if relu else None
+    out = torch.empty(dy.shape, dtype=torch.bfloat16, device=dy.device)
+    db = torch.zeros(c, dtype=torch.float32, device=dy.device)
+    _check(_lib().ggh_relu_bias_backward(dy.data_ptr(), int(dy.dtype == torch.float32),
+                                         yc.data_ptr() if relu else None, int(bool(relu)),
+                                         out.data_ptr(), db.data_ptr(), dy.numel() // c, c, _stream(dy)))
+    return out, db
+
+
 def flip_transpose(w):
     """weights of the data gradient: [cout][9][cin] -> [cin][9][cout], taps reversed"""
     cout, _, cin = w.shape
@@ -108,14 +126,11 @@ class _Conv3x3(torch.autograd.Function):
     def backward(ctx, dy):
         x, wb, y = ctx.saved_tensors
         cout, _, cin = wb.shape
-        dy = dy.to(torch.bfloat16).contiguous()
-        if ctx.relu:
-            dy = dy * (y > 0).to(torch.bfloat16)
+        dy, db = relu_bias_backward(dy, y, ctx.relu)
         dx = None
         if ctx.needs_input_grad[0]:
             dx = conv3x3(dy, flip_transpose(wb).view(cin, 9 * cout))
         dw = conv3x3_weight_grad(x, dy).view(cout, 9, cin)
-        db = dy.float().sum(dim=(0, 1, 2))
         return dx, dw, db, None
 
 

@@ -273,6 +273,12 @@ int ggh_conv3x3_forward(const void* x, const void* w, const float* bias, void* y
 int ggh_conv3x3_weight_grad(const void* x, const void* dy, float* dw, int32_t n, int32_t h, int32_t w_px, int32_t cin,
                             int32_t cout, int32_t splits, void* stream);
 
+/* backward of a layer's ReLU and bias: dy_out = dy where y > 0 (everywhere
+ * when relu == 0), BF16; db[c] += sum over the pixels of dy_out, FP32.
+ * dy is BF16, or FP32 when dy_is_f32; all NHWC with c one of 8, 16, 32, 64, 128 */
+int ggh_relu_bias_backward(const void* dy, int32_t dy_is_f32, const void* y, int32_t relu, void* dy_out, float* db,
+                           int64_t n_pixels, int32_t c, void* stream);
+
 /* ---- test entry points -------------------------------------------------- */
 /* voxel indices visited by rays crossing voxel solid `solid` with no
  * interaction; pos/dir are global.  out_index[r*max_steps + k], out_count[r]. */

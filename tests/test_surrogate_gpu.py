@@ -51,6 +51,21 @@ def test_conv_gradients_match_fp32(n, h, w, cin, cout):
     assert torch.all((dx - xr.grad).abs() <= 2 ** -7 * xr.grad.abs() + 1e-4), float((dx - xr.grad).abs().max())
 
 
+@pytest.mark.parametrize("c,relu,f32", [(16, True, False), (64, False, False), (128, True, True), (32, True, True)])
+def test_relu_bias_backward_matches_torch(c, relu, f32):
+    """The fused ReLU/bias backward against the same steps in torch on the CPU:
+    the mask is exact, the bias gradient an FP32 sum in another order."""
+    g = torch.Generator().manual_seed(c + 3 * relu)
+    dy = torch.randn(3, 40, 48, c, generator=g)
+    dy = dy if f32 else dy.bfloat16()
+    y = torch.randn(3, 40, 48, c, generator=g).bfloat16()
+    got, db = S.relu_bias_backward(dy.cuda(), y.cuda(), relu)
+    want = dy.bfloat16() * ((y > 0).bfloat16() if relu else 1)
+    assert got.dtype == torch.bfloat16 and torch.equal(got.cpu(), want)
+    ref = want.double().sum(dim=(0, 1, 2))
+    assert torch.allclose(db.cpu().double(), ref, rtol=1e-5, atol=1e-3), float((db.cpu().double() - ref).abs().max())
+
+
 def test_unet_forward_and_gradients_match_the_fp32_reference():
     torch.manual_seed(0)
     net = S.UNet(seed=5).cuda()
