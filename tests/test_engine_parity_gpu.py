@@ -151,6 +151,32 @@ def test_batch_layout_edge_cases_match_oracle(cfg1, n):
     eng.close()
 
 
+def test_photon_tracking_dose_matches_oracle():
+    """Dosimetry with photon tracking (the fluence tally at each transport
+    step's point of departure, dose_photon_tracking,
+    include/GGEMS/navigators/GGEMSDoseRecording.hh:99-118), which runs the
+    general kernel: tallies against the oracle's, history by history."""
+    sc = Scene()
+    sc.add_phantom(cylinder_phantom(60, radius=12.0), ["Water", "RibBone"], (1.0, 1.0, 1.0),
+                   dosimetry={"photon_tracking": True})
+    sc.add_ct_flat("GOS", (1, 1), (96, 96, 1), (1.0, 1.0, 1.0), 600.0, 1000.0, 10e-3)
+    sc.set_source((-600.0, 0.0, 0.0), 3.0, [0.05], mono=True)
+    sc.build()
+    n = 100_000
+    ref = sc.run_oracle(n, 606, threads=0)
+    eng = sc.make_engine()
+    got = sc.run_engine(n, 606, eng)
+    assert eng.last_grid()[2] == 1
+    pt_ref, pt_got = ref["photon_tracking"], got["photon_tracking"]
+    assert pt_ref.sum() > 10 * n  # every transport step is counted
+    assert abs(int(pt_got.sum()) - int(pt_ref.sum())) <= 0.01 * pt_ref.sum()
+    assert np.abs(pt_ref.astype(np.int64) - pt_got).sum() <= 0.02 * pt_ref.sum()
+    assert np.abs(ref["hit"].astype(np.int64) - got["hit"]).sum() <= 0.02 * ref["hit"].sum()
+    assert abs(got["edep"].sum() - ref["edep"].sum()) <= 0.01 * ref["edep"].sum()
+    assert abs(got["edep2"].sum() - ref["edep2"].sum()) <= 0.02 * ref["edep2"].sum()
+    eng.close()
+
+
 def test_two_phantoms_match_oracle():
     """Two voxel phantoms side by side (one of them rotated), so that one wave
     holds photons in different voxel solids: the general kernel's regrouping
