@@ -325,9 +325,10 @@ __global__ __launch_bounds__(64 * kWaves) void conv_wgrad_kernel(const __bf16* _
 // the layer's output y > 0 (everywhere without ReLU), as BF16, and
 // db[c] += sum over pixels of dy_out[., c] in FP32.  A thread takes 8
 // channels of a pixel; the grid's thread count is a multiple of C / 8, so a
-// thread keeps the same 8 channels over its whole grid-stride loop, sums
-// them in registers, then adds them into the block's LDS sums, which one
-// global atomic per channel and block finishes.
+// thread keeps the same 8 channels over its whole grid-stride loop and sums
+// them in registers; the wave folds those sums onto its first C / 8 lanes,
+// which add them into the block's LDS sums, and one global atomic per
+// channel and block finishes.
 constexpr int kMaskBlock = 256;
 
 template <bool kF32>
@@ -365,7 +366,13 @@ __global__ __launch_bounds__(kMaskBlock) void relu_bias_backward_kernel(const vo
     reinterpret_cast<bf16x8*>(dy_out)[q] = o;
     for (int e = 0; e < 8; ++e) acc[e] += (float)o[e];
   }
-  for (int e = 0; e < 8; ++e) atomicAdd(&s_db[c8 + e], acc[e]);
+  // lanes l and l xor off (off a multiple of C / 8, a power of two) hold the
+  // same channels: fold the wave onto its first C / 8 lanes, which then add
+  // into the block's sums
+  for (int off = 32; off >= groups; off >>= 1)
+    for (int e = 0; e < 8; ++e) acc[e] += __shfl_xor(acc[e], off, 64);
+  if ((int)(threadIdx.x & 63) < groups)
+    for (int e = 0; e < 8; ++e) atomicAdd(&s_db[c8 + e], acc[e]);
   __syncthreads();
   for (int i = threadIdx.x; i < C; i += kMaskBlock) unsafeAtomicAdd(&db[i], s_db[i]);
 }
