@@ -177,6 +177,35 @@ def test_photon_tracking_dose_matches_oracle():
     eng.close()
 
 
+@pytest.mark.parametrize("general", [False, True])
+def test_launch_shape_does_not_change_histories(cfg1, general):
+    """Rounds of 1 or 1,024 steps, 1 to 16 navigation passes, interaction
+    batches of 1 to 64 lanes: the regrouping only moves photons between
+    lanes, so every history, and with it every count, is the same."""
+    n = 150_000
+    eng = cfg1.make_engine()
+    eng.set_launch(general_kernel=general)
+    base = cfg1.run_engine(n, 2024, eng)
+    for rs, cp, ib in [(1, 1, 1), (1024, 16, 64), (7, 3, 5)]:
+        eng.set_launch(round_steps=rs, cold_passes=cp, interact_batch=ib, general_kernel=general)
+        got = cfg1.run_engine(n, 2024, eng)
+        assert got["stats"] == base["stats"], (rs, cp, ib)
+        assert np.array_equal(got["hist"][0], base["hist"][0]) and np.array_equal(got["hit"], base["hit"])
+        # FP64 sums whose order follows the atomics
+        np.testing.assert_allclose(got["edep"], base["edep"], rtol=1e-12, atol=1e-18)
+    eng.close()
+
+
+def test_launch_rejects_invalid_parameters(cfg1):
+    from ggems_amd.engine import EngineError
+    eng = cfg1.make_engine()
+    for kw in (dict(round_steps=-1), dict(round_steps=1025), dict(cold_passes=17), dict(interact_batch=65),
+               dict(blocks_per_cu=-2)):
+        with pytest.raises(EngineError):
+            eng.set_launch(**kw)
+    eng.close()
+
+
 def test_two_phantoms_match_oracle():
     """Two voxel phantoms side by side (one of them rotated), so that one wave
     holds photons in different voxel solids: the general kernel's regrouping
