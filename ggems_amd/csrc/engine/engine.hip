@@ -1986,6 +1986,12 @@ namespace {
 // consecutive triangles.  Boxes are widened by 1e-3 mm plus 1e-5 of their
 // coordinates so that rounding in the device's slab test cannot prune a box
 // whose triangles the exact intersection test would hit.
+// Triangles per BVH leaf.  One measured fastest on the mesh-CT scene (164 M
+// primaries/s, against 157 with two, 152 with four and 137 with eight): a
+// Moller-Trumbore test costs several node tests, so it pays to run it only
+// for triangles whose own box the ray crosses.
+constexpr int kBvhLeaf = 1;
+
 struct BvhBuild {
   const float* tri;
   std::vector<int> order;
@@ -2016,7 +2022,7 @@ struct BvhBuild {
       hi[a] += w;
     }
     int info = 0;
-    if (e - b <= 4) {
+    if (e - b <= kBvhLeaf) {
       info = (b << 4) | (e - b);
     } else {
       float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
