@@ -372,7 +372,7 @@ void set_rotation_ggems_voxelized_phantom(GGEMSVoxelizedPhantom* p, float x, flo
   API_TRY p->set_rotation(x, y, z, str(u));
   API_CATCH()
 }
-// OpenGL display settings: accepted, no display in this build
+// OpenGL display settings: accepted; this library has no display
 void set_visible_ggems_voxelized_phantom(GGEMSVoxelizedPhantom*, bool) {}
 void set_material_visible_ggems_voxelized_phantom(GGEMSVoxelizedPhantom*, char const*, bool) {}
 void set_material_color_ggems_voxelized_phantom(GGEMSVoxelizedPhantom*, char const*, unsigned char, unsigned char,
@@ -398,7 +398,7 @@ void set_material_name_ggems_meshed_phantom(GGEMSMeshedPhantom* p, char const* m
   API_CATCH()
 }
 void set_mesh_octree_depth_ggems_meshed_phantom(GGEMSMeshedPhantom* p, int depth) { p->octree_depth = depth; }
-// OpenGL display settings: accepted, no display in this build
+// OpenGL display settings: accepted; this library has no display
 void set_visible_ggems_meshed_phantom(GGEMSMeshedPhantom*, bool) {}
 void set_material_color_ggems_meshed_phantom(GGEMSMeshedPhantom*, char const*, unsigned char, unsigned char,
                                              unsigned char) {}
@@ -488,7 +488,7 @@ void dose_hit_dosimetry_calculator(GGEMSDosimetryCalculator* d, bool f) { d->hit
 void dose_edep_squared_dosimetry_calculator(GGEMSDosimetryCalculator* d, bool f) { d->edep_squared = f; }
 void dose_uncertainty_dosimetry_calculator(GGEMSDosimetryCalculator* d, bool f) { d->uncertainty = f; }
 void dose_tle_navigator(GGEMSDosimetryCalculator*, bool f) {
-  API_TRY if (f) raise("GGEMSDosimetryCalculator", "track-length estimator (TLE) dose is not available in this build");
+  API_TRY if (f) raise("GGEMSDosimetryCalculator", "track-length estimator (TLE) dose is not available in this library");
   API_CATCH()
 }
 void attach_to_navigator_dosimetry_calculator(GGEMSDosimetryCalculator* d, char const* name) {
