@@ -50,20 +50,9 @@ def algorithmic_bytes_per_primary(stats):
 def cpu_baseline(workdir, n_per_step, seed, target_s, threads):
     """The CPU oracle on a bounded sample of particle slots of the same workload."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
-    import math
+    from scene import voxel_ct_scene
 
-    import numpy as np
-    from scene import Scene
-
-    from ggems_amd import authoring, configs
-    mhd, rng_file, spec = configs.voxel_ct_inputs(workdir)
-    lab, spacing = authoring.read_mhd(mhd)
-    e, w = np.loadtxt(spec, unpack=True)
-    sc = Scene()
-    sc.add_phantom(lab, authoring.BODY_MATERIALS, tuple(spacing), dosimetry={"photon_tracking": False})
-    sc.add_ct_flat("GOS", (1, 1), (256, 256, 1), (1.0, 1.0, 1.0), 600.0, 1000.0, 10e-3)
-    sc.set_source((-600.0, 0.0, 0.0), math.degrees(math.atan(128 * math.sqrt(2.0) / 1000.0)), e, intensities=w)
-    sc.build()
+    sc = voxel_ct_scene(workdir)
     prepared = sc.prepare_oracle()  # tallies allocated and faulted in outside the timed region
     # consecutive chunks of particle slots until about target_s seconds of CPU work
     begin, chunk, done, elapsed = 0, 2048, 0, 0.0

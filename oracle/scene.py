@@ -444,3 +444,19 @@ def cfg1_scene(n_side=100, detector=256, dosimetry=True, energy_kev=60.0, materi
         aperture_deg = math.degrees(math.atan(detector * 0.5 * math.sqrt(2.0) / sdd))
     sc.set_source((-sid, 0.0, 0.0), aperture_deg, [energy_kev * 1e-3], mono=True)
     return sc.build()
+
+
+def voxel_ct_scene(workdir):
+    """The voxel-CT workload: the 512x512x256 body phantom of 0.8 mm voxels
+    (five materials, FP64 dosimetry on the voxel grid), the 120 kVp spectrum
+    behind 2.5 mm Al, a flat 256x256 GOS detector of 1 mm pixels at SDD
+    1000 mm and a point cone beam at SID 600 mm that just covers it."""
+    from ggems_amd import authoring, configs
+    mhd, _, spec = configs.voxel_ct_inputs(workdir)
+    lab, spacing = authoring.read_mhd(mhd)
+    e, w = np.loadtxt(spec, unpack=True)
+    sc = Scene()
+    sc.add_phantom(lab, authoring.BODY_MATERIALS, tuple(spacing), dosimetry={"photon_tracking": False})
+    sc.add_ct_flat("GOS", (1, 1), (256, 256, 1), (1.0, 1.0, 1.0), 600.0, 1000.0, 10e-3)
+    sc.set_source((-600.0, 0.0, 0.0), math.degrees(math.atan(128 * math.sqrt(2.0) / 1000.0)), e, intensities=w)
+    return sc.build()
