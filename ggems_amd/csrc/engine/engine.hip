@@ -1683,10 +1683,14 @@ int check_ready(ggh_context* c) {
 
 // GGEMSSource::OrganizeParticlesInBatch (src/sources/GGEMSSource.cc:403-425):
 // ceil(float(n) / slots) batches of n / batches photons, the first n % batches
-// batches one more; slots = the largest batch
+// batches one more; slots = the largest batch.  Above 2^24 primaries the
+// float quotient can round n down by enough to give one batch too few (2^24 + 1
+// primaries: 16 batches, the first of 2^20 + 1 photons), whose largest batch
+// would overrun the particle buffer; such n take the exact ceiling instead.
 int batch_layout(uint64_t n, uint32_t& n_batches, uint32_t& base, uint32_t& rem, uint32_t& slots) {
   uint64_t nb = (uint64_t)std::ceil((float)n / (float)GGH_MAX_SLOTS);
   if (nb == 0) nb = 1;
+  if (nb * GGH_MAX_SLOTS < n) nb = (n + GGH_MAX_SLOTS - 1) / GGH_MAX_SLOTS;
   const uint64_t b = nb == 1 ? n : n / nb;
   const uint64_t r = nb == 1 ? 0 : n % nb;
   const uint64_t sl = b + (r ? 1 : 0);

@@ -742,9 +742,14 @@ static void transport_primary(const ggo_physics* ph, const ggo_solid* solids, in
   }
 }
 
+/* GGEMSSource::OrganizeParticlesInBatch (src/sources/GGEMSSource.cc:403-425):
+ * ceil(float(n) / slots) batches.  Where the float quotient gives one batch
+ * too few (n above 2^24 that rounds down, e.g. 2^24 + 1), the largest batch
+ * would exceed the slot buffer, and the exact ceiling is taken instead. */
 void ggo_batch_layout(uint64_t n, uint64_t* nb, uint64_t* ns) {
   uint64_t b = (uint64_t)ceilf((float)n / (float)MAX_SLOTS);
   if (b == 0) b = 1;
+  if (b * MAX_SLOTS < n) b = (n + MAX_SLOTS - 1) / MAX_SLOTS;
   *nb = b;
   *ns = b == 1 ? n : n / b + (n % b ? 1 : 0);
 }

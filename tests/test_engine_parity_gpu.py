@@ -129,12 +129,14 @@ def test_cfg1_run_matches_oracle(cfg1, general, interact_batch):
     eng.close()
 
 
-@pytest.mark.parametrize("n", [1, 3, 1_048_576 + 4_097])
+@pytest.mark.parametrize("n", [1, 3, 1_048_576 + 4_097, (1 << 24) + 1])
 def test_batch_layout_edge_cases_match_oracle(cfg1, n):
     """Runs of 1 and 3 primaries, and one just over a batch: GGEMS splits it
     into two batches of 526,337 and 526,336 photons
     (src/sources/GGEMSSource.cc:403-425), so slot 526,336 carries one photon
-    and every lower slot two."""
+    and every lower slot two.  At 2^24 + 1 primaries the float ceiling of the
+    batch count gives 16 batches whose first would overrun the slots; the
+    exact ceiling gives 17, two of 986,896 photons and 15 of 986,895."""
     ref = cfg1.run_oracle(n, 4242, threads=0)
     eng = cfg1.make_engine()
     got = cfg1.run_engine(n, 4242, eng)
@@ -175,6 +177,21 @@ def test_photon_tracking_dose_matches_oracle():
     assert abs(got["edep"].sum() - ref["edep"].sum()) <= 0.01 * ref["edep"].sum()
     assert abs(got["edep2"].sum() - ref["edep2"].sum()) <= 0.02 * ref["edep2"].sum()
     eng.close()
+
+
+def test_runs_beyond_2_32_primaries_count_every_one(cfg1):
+    """2^32 + 12,345 primaries in one run: 4,097 batches, counted in 64-bit
+    totals, with the detector counts per primary of an independent shorter
+    run to within 5 Poisson errors."""
+    n, m = (1 << 32) + 12_345, 1 << 26
+    eng = cfg1.make_engine()
+    big = cfg1.run_engine(n, 99, eng)
+    small = cfg1.run_engine(m, 100, eng)
+    eng.close()
+    assert big["stats"]["primaries"] == n and small["stats"]["primaries"] == m
+    cb, cs = int(big["hist"][0].astype(np.int64).sum()), int(small["hist"][0].astype(np.int64).sum())
+    assert cs > 10_000 and big["hist"][0].min() >= 0
+    assert abs(cb / n - cs / m) <= 5.0 * np.sqrt(cs) / m, (cb / n, cs / m)
 
 
 @pytest.mark.parametrize("general", [False, True])

@@ -389,3 +389,28 @@ def test_dose_and_uncertainty_closed_forms():
     O.lib().ggo_compute_dose(C.byref(solids[0]), O.ptr(dens), F(1.0), 0, F(2.0 * 6.241509704e21 / 1e3), O.ptr(dose),
                              O.ptr(unc))
     assert not dose.any()
+
+
+def test_batch_layout_follows_the_float_ceiling_and_fits_the_slot_buffer():
+    """GGEMSSource::OrganizeParticlesInBatch takes ceil(float(n) / 2^20)
+    batches (src/sources/GGEMSSource.cc:403-425).  Wherever its largest batch
+    fits the 2^20 slots the layout is the reference's; where float rounding
+    leaves one batch too few (2^24 + 1 primaries gives 16 batches, the first
+    of 2^20 + 1) the exact ceiling is taken."""
+    slots = 1 << 20
+    rng = np.random.default_rng(11)
+    ns = [1, 3, slots, slots + 1, 1_000_000_000, (1 << 24) + 1, (1 << 25) + 7, (1 << 32) + 12_345, 10 ** 12]
+    ns += [int(v) for v in rng.integers(1, 1 << 40, 2000)]
+    ns += [int(k) * slots + int(d) for k, d in zip(rng.integers(16, 1 << 20, 500), rng.integers(1, 64, 500))]
+    n_exact = 0
+    for n in ns:
+        nb, largest = O.batch_layout(n)
+        ref = max(1, int(np.ceil(np.float32(n) / np.float32(slots))))
+        assert nb * slots >= n and largest <= slots and largest == -(-n // nb), n
+        if ref * slots >= n:
+            assert nb == ref, n
+        else:
+            n_exact += 1
+            assert nb == -(-n // slots), n
+    assert O.batch_layout((1 << 24) + 1) == (17, -(-((1 << 24) + 1) // 17))
+    assert n_exact > 100  # the rounding case is common above 2^24
