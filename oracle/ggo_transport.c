@@ -742,6 +742,18 @@ static void transport_primary(const ggo_physics* ph, const ggo_solid* solids, in
   }
 }
 
+/* Worker threads when the caller passes 0: OMP_NUM_THREADS when set (the
+ * machine's CPU share), else the online processors, at most 16. */
+static int default_threads(void) {
+  const char* e = getenv("OMP_NUM_THREADS");
+  int t = e ? atoi(e) : 0;
+  if (t <= 0) {
+    t = (int)sysconf(_SC_NPROCESSORS_ONLN);
+    if (t > 16) t = 16;
+  }
+  return t < 1 ? 1 : t;
+}
+
 /* GGEMSSource::OrganizeParticlesInBatch (src/sources/GGEMSSource.cc:403-425):
  * ceil(float(n) / slots) batches.  Where the float quotient gives one batch
  * too few (n above 2^24 that rounds down, e.g. 2^24 + 1), the largest batch
@@ -793,7 +805,7 @@ uint64_t ggo_run_slots(const ggo_physics* ph, const ggo_solid* solids, int ns, c
   ggo_seed_slots(seed, seeded, states);
   if (slot_end > nslots) slot_end = nslots;
   if (slot_begin > slot_end) slot_begin = slot_end;
-  if (threads <= 0) threads = (int)sysconf(_SC_NPROCESSORS_ONLN);
+  if (threads <= 0) threads = default_threads();
   if (threads < 1) threads = 1;
   worker* ws = (worker*)calloc((size_t)threads, sizeof(worker));
   pthread_t* th = (pthread_t*)calloc((size_t)threads, sizeof(pthread_t));
@@ -914,7 +926,7 @@ int ggo_run_acquisition(const ggo_physics* ph, const ggo_solid* solids, int ns, 
   int n_box = 0;
   for (int k = 0; k < ns; ++k) n_box += solids[k].type == GGO_SOLID_BOX;
   ggo_seed_slots(seed, MAX_SLOTS, states);
-  if (threads <= 0) threads = (int)sysconf(_SC_NPROCESSORS_ONLN);
+  if (threads <= 0) threads = default_threads();
   if (threads < 1) threads = 1;
   acq_worker* ws = (acq_worker*)calloc((size_t)threads, sizeof(acq_worker));
   pthread_t* th = (pthread_t*)calloc((size_t)threads, sizeof(pthread_t));
