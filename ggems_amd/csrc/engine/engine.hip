@@ -139,6 +139,7 @@ struct RunParams {
   int32_t round_steps;        // voxel steps a wave makes at least per round
   int32_t cold_passes;        // navigation passes per round
   int32_t interact_batch;
+  uint32_t counter_bits;      // lane counters are flushed once one reaches 2^counter_bits
   uint32_t xchg_offset;       // byte offset of the lane-state exchange area in dynamic LDS
   VoxGeom vox0;               // the only voxel solid, when there is exactly one
   // CT acquisition: projection p places the source at acq_src[p] and box
@@ -786,9 +787,32 @@ __device__ int find_and_project(const RunParams& P, const Tabs& T, f3& pos, f3 d
   return solid;
 }
 
-__device__ inline uint32_t wave_sum(uint32_t v) {
+__device__ inline unsigned long long wave_sum(unsigned long long v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
+}
+
+// The lane counters are 32-bit (registers are the kernel's scarce resource).
+// When a lane starts a primary and one of its counters has reached
+// 2^counter_bits (2^31 unless a test lowers it), the lane adds all of them to
+// the 64-bit run counters and clears them.  One history adds far less than
+// 2^31 to any counter, so none can wrap.  At the end each wave adds its
+// lanes' remainders (sums of 64 values below 2^32, in 64 bits).
+__device__ inline void flush_lane_counters(const RunParams& P, Counters& c) {
+  const uint32_t v[5] = {c.primaries, c.interactions, c.detector_steps, c.detector_interactions, c.hops};
+  const int slots[5] = {0, 2, 3, 4, 5};
+  for (int k = 0; k < 5; ++k)
+    if (v[k]) atomicAdd(&P.stats[slots[k]], (unsigned long long)v[k]);
+  c = Counters{0, 0, 0, 0, 0};
+}
+
+__device__ inline void flush_wave_counters(const RunParams& P, const Counters& c) {
+  const unsigned long long sums[5] = {wave_sum(c.primaries), wave_sum(c.interactions),
+                                      wave_sum(c.detector_steps), wave_sum(c.detector_interactions),
+                                      wave_sum(c.hops)};
+  const int slots[5] = {0, 2, 3, 4, 5};
+  for (int k = 0; k < 5; ++k)
+    if (__lane_id() == 0 && sums[k]) atomicAdd(&P.stats[slots[k]], sums[k]);
 }
 
 __device__ inline void load_stream(const RunParams& P, uint32_t slot, Rng& r) {
@@ -897,6 +921,9 @@ __device__ inline void cold_step(const RunParams& P, const Tabs& T, Lane& L, Cou
       L.mode = kNeedSlot;  // empty slot: take another one on the next pass
     } else {
       --L.todo;
+      if (((c.primaries | c.interactions | c.detector_steps | c.detector_interactions | c.hops) >>
+           P.counter_bits) != 0u)
+        flush_lane_counters(P, c);
       sample_primary<kAcq>(P, T, kAcq ? P.acq_src[L.proj] : P.src_m, L.r, L.p, L.ray.d, L.e, L.eid);
       ++c.primaries;
       L.scatter = 0;
@@ -1137,12 +1164,7 @@ void transport_kernel(RunParams P) {
     }
   }
 
-  uint32_t sums[5] = {c.primaries, c.interactions, c.detector_steps, c.detector_interactions, c.hops};
-  const int slots[5] = {0, 2, 3, 4, 5};
-  for (int k = 0; k < 5; ++k) {
-    uint32_t t = wave_sum(sums[k]);
-    if (__lane_id() == 0 && t) atomicAdd(&P.stats[slots[k]], (unsigned long long)t);
-  }
+  flush_wave_counters(P, c);
   if (__lane_id() == 0 && steps) atomicAdd(&P.stats[1], steps);
 }
 
@@ -1416,6 +1438,7 @@ int fail(int code, const std::string& msg) {
       return fail(-100 - (int)e_, std::string(#expr) + ": " + hipGetErrorString(e_));   \
   } while (0)
 
+
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
@@ -1499,6 +1522,7 @@ struct ggh_context {
   int cold_passes = 2;
   int interact_batch = 16;
   int blocks_per_cu = 0;  // 0: as many as fit
+  int counter_bits = 31;
   bool force_general = false;
   unsigned last_grid = 0;
   int last_variant = -1;
@@ -1617,6 +1641,7 @@ RunParams make_params(ggh_context* c) {
   P.round_steps = c->round_steps;
   P.cold_passes = c->cold_passes;
   P.interact_batch = c->interact_batch;
+  P.counter_bits = (uint32_t)c->counter_bits;
   P.xchg_offset = 0;
   for (const DevSolid& s : c->solids)
     if (s.type == kVoxel) {
@@ -1885,6 +1910,8 @@ int ggh_set_physics(ggh_context* c, const ggh_physics_desc* d) {
   c->n_mat = nm;
   c->n_proc = d->n_processes;
   for (int i = 0; i < 3; ++i) c->proc[i] = i < d->n_processes ? d->process_id[i] : 0;
+  // the captured acquisition holds the table pointers reallocated below
+  drop_graph(c);
   c->factor = d->rayleigh_factor;
   c->density_host.assign(d->density, d->density + nm);
   c->ebins_host.assign(d->energy_bins, d->energy_bins + nb);
@@ -1936,6 +1963,7 @@ int ggh_add_voxel_solid(ggh_context* c, const ggh_voxel_solid_desc* d) {
   c->mesh_bvh.push_back(DevBuf<float4>());
   c->mesh_tris.push_back(DevBuf<float4>());
   c->hist_elems.push_back(0);
+  drop_graph(c);  // a captured acquisition was recorded for the previous solid list
   c->solids.push_back(s);
   c->max_material.push_back(d->material_offset + max_label);
   c->solids_dirty = true;
@@ -1975,6 +2003,7 @@ int ggh_add_box_solid(ggh_context* c, const ggh_box_solid_desc* d) {
   c->mesh_bvh.push_back(DevBuf<float4>());
   c->mesh_tris.push_back(DevBuf<float4>());
   c->hist_elems.push_back(ne);
+  drop_graph(c);  // a captured acquisition was recorded for the previous solid list
   c->solids.push_back(s);
   c->max_material.push_back(d->material_offset);
   c->solids_dirty = true;
@@ -2175,6 +2204,7 @@ int ggh_add_mesh_solid(ggh_context* c, const ggh_mesh_solid_desc* d) {
   c->mesh_bvh.push_back(bvh);
   c->mesh_tris.push_back(tb);
   c->hist_elems.push_back(0);
+  drop_graph(c);  // a captured acquisition was recorded for the previous solid list
   c->solids.push_back(s);
   c->max_material.push_back(d->material_offset);
   c->solids_dirty = true;
@@ -2213,6 +2243,7 @@ int ggh_set_dosimetry(ggh_context* c, const ggh_dosimetry_desc* d) {
   size_t n = (size_t)d->n_dosels[0] * d->n_dosels[1] * d->n_dosels[2];
   if (d->n_dosels[0] <= 0 || d->n_dosels[1] <= 0 || d->n_dosels[2] <= 0 || n > (size_t)INT32_MAX)
     return fail(-2, "invalid dosel grid");
+  drop_graph(c);  // the captured acquisition holds the tally pointers reallocated below
   c->dose_desc = *d;
   if ((rc = dev_alloc(c->edep, n))) return rc;
   c->edep2.release();
@@ -2229,6 +2260,7 @@ int ggh_set_xray_source(ggh_context* c, const ggh_xray_source_desc* d) {
   int rc = check_ctx(c);
   if (rc) return rc;
   if (!d || d->n_energy < 1 || !d->energy || !d->cdf) return fail(-2, "invalid source description");
+  drop_graph(c);  // the captured acquisition holds the spectrum and cdf reallocated below
   c->src_m = to_mat34(d->matrix);
   c->aperture = d->aperture;
   c->focal = mk(d->focal_spot[0], d->focal_spot[1], d->focal_spot[2]);
@@ -2291,13 +2323,15 @@ int ggh_set_launch(ggh_context* c, const ggh_launch_desc* d) {
   int rc = check_ctx(c);
   if (rc) return rc;
   if (!d || d->round_steps < 0 || d->round_steps > 1024 || d->cold_passes < 0 || d->cold_passes > 16 ||
-      d->interact_batch < 0 || d->interact_batch > 64 || d->blocks_per_cu < 0 || d->blocks_per_cu > 64)
+      d->interact_batch < 0 || d->interact_batch > 64 || d->blocks_per_cu < 0 || d->blocks_per_cu > 64 ||
+      d->counter_bits < 0 || d->counter_bits > 31)
     return fail(-2, "invalid launch parameters");
   c->round_steps = d->round_steps ? d->round_steps : 20;
   c->cold_passes = d->cold_passes ? d->cold_passes : 2;
   c->interact_batch = d->interact_batch ? d->interact_batch : 16;
   c->blocks_per_cu = d->blocks_per_cu;
   c->force_general = d->general_kernel != 0;
+  c->counter_bits = d->counter_bits ? d->counter_bits : 31;
   return 0;
 }
 
@@ -2445,11 +2479,13 @@ int host_bin(const std::vector<float>& a, float key) {
 // Probability of each energy bin of the cross-section tables under the
 // source's sampling (GetPrimariesGGEMSXRaySource.cl:103-125): line idx has
 // probability cdf[idx+1] - cdf[idx]; an interpolated spectrum spreads it
-// uniformly over [E_idx, E_idx+1), split at the bin edges.
-void drr_weights(const ggh_context* c, std::vector<int32_t>& bins, std::vector<float>& w) {
+// uniformly over [E_idx, E_idx+1), split at the bin edges.  W holds nbins
+// (<= 2048) doubles of work space; bins and w receive the bins of nonzero
+// weight, whose count is returned.  Allocates nothing.
+int drr_weights(const ggh_context* c, double* W, int32_t* bins, float* w) {
   const std::vector<float>& eb = c->ebins_host;
   const int nb = (int)eb.size();
-  std::vector<double> W((size_t)nb, 0.0);
+  for (int k = 0; k < nb; ++k) W[k] = 0.0;
   const std::vector<float>& e = c->spectrum_host;
   const std::vector<float>& f = c->cdf_host;
   for (int idx = 0; idx + 1 < (int)f.size() && idx + 1 < (int)e.size(); ++idx) {
@@ -2467,20 +2503,20 @@ void drr_weights(const ggh_context* c, std::vector<int32_t>& bins, std::vector<f
       if (ov > 0.0) W[k] += p * ov / (b - a);
     }
   }
-  bins.clear();
-  w.clear();
+  int n = 0;
   for (int k = 0; k < nb; ++k)
     if (W[k] > 0.0) {
-      bins.push_back(k);
-      w.push_back((float)W[k]);
+      bins[n] = k;
+      w[n] = (float)W[k];
+      ++n;
     }
+  return n;
 }
 
-}  // namespace
-
-extern "C" {
-
-int ggh_drr(ggh_context* c, int solid, int projection, float* image) {
+// Checks shared by both ray-tracer entry points; fills the launch
+// parameters except the weights and the image.  Allocates nothing when the
+// solid table is already on the device (allow_upload false).
+int drr_setup(ggh_context* c, int solid, int projection, const float* image, bool allow_upload, DrrParams& D) {
   int rc = check_ctx(c);
   if (rc) return rc;
   if (!image) return fail(-1, "null image");
@@ -2491,8 +2527,11 @@ int ggh_drr(ggh_context* c, int solid, int projection, float* image) {
   for (size_t i = 0; i < c->solids.size(); ++i)
     if (c->solids[i].type == kVoxel && c->max_material[i] >= c->n_mat)
       return fail(-3, "solid labels reach a material without physics tables");
-  if ((rc = sync_solids(c))) return rc;
-  DrrParams D;
+  if (c->solids_dirty) {
+    if (!allow_upload)
+      return fail(-3, "the solid table is not on the device yet: a run or ggh_drr uploads it after a solid is added");
+    if ((rc = sync_solids(c))) return rc;
+  }
   std::memset(&D, 0, sizeof D);
   const DevSolid& box = c->solids[solid];
   if (projection < 0) {
@@ -2504,24 +2543,38 @@ int ggh_drr(ggh_context* c, int solid, int projection, float* image) {
     D.src_m = c->acq_src_host[projection];
     D.box_m = c->acq_box_host[(size_t)projection * n_box + box.box_index];
   }
-  std::vector<int32_t> bins;
-  std::vector<float> w;
-  drr_weights(c, bins, w);
-  const size_t npx = (size_t)box.ne[0] * box.ne[1];
-  DevBuf<int32_t> db;
-  DevBuf<float> dw, dimg;
-  if ((rc = dev_upload(db, bins.data(), bins.size(), c->stream)) || (rc = dev_upload(dw, w.data(), w.size(), c->stream)) ||
-      (rc = dev_alloc(dimg, npx)))
-    return rc;
   D.solids = c->solids_dev.p;
   D.n_solids = (int32_t)c->solids.size();
   D.box = solid;
   D.sig = c->tab_sig.p;
   D.nbins = c->nbins;
   D.n_mat = c->n_mat;
+  return 0;
+}
+
+
+}  // namespace
+
+extern "C" {
+
+int ggh_drr(ggh_context* c, int solid, int projection, float* image) {
+  DrrParams D;
+  int rc = drr_setup(c, solid, projection, image, true, D);
+  if (rc) return rc;
+  const DevSolid& box = c->solids[solid];
+  std::vector<double> W((size_t)c->nbins);
+  std::vector<int32_t> bins((size_t)c->nbins);
+  std::vector<float> w((size_t)c->nbins);
+  const int n_w = drr_weights(c, W.data(), bins.data(), w.data());
+  const size_t npx = (size_t)box.ne[0] * box.ne[1];
+  DevBuf<int32_t> db;
+  DevBuf<float> dw, dimg;
+  if ((rc = dev_upload(db, bins.data(), (size_t)n_w, c->stream)) || (rc = dev_upload(dw, w.data(), (size_t)n_w, c->stream)) ||
+      (rc = dev_alloc(dimg, npx)))
+    return rc;
   D.w_bin = db.p;
   D.w = dw.p;
-  D.n_w = (int32_t)bins.size();
+  D.n_w = n_w;
   D.image = dimg.p;
   hipLaunchKernelGGL(drr_kernel, dim3((unsigned)((npx + kDrrBlock - 1) / kDrrBlock)), dim3(kDrrBlock), 0, c->stream, D);
   hipError_t e = hipGetLastError();

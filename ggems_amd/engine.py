@@ -66,7 +66,7 @@ class XRaySourceDesc(C.Structure):
 
 class LaunchDesc(C.Structure):
     _fields_ = [("round_steps", C.c_int32), ("cold_passes", C.c_int32), ("interact_batch", C.c_int32),
-                ("blocks_per_cu", C.c_int32), ("general_kernel", C.c_int32)]
+                ("blocks_per_cu", C.c_int32), ("general_kernel", C.c_int32), ("counter_bits", C.c_int32)]
 
 
 class AcquisitionDesc(C.Structure):
@@ -398,10 +398,12 @@ class Engine:
         _check(library().ggh_debug_primaries(self.h, n, _ptr(pos), _ptr(dir), _ptr(e)))
         return pos, dir, e
 
-    def set_launch(self, round_steps=0, interact_batch=0, blocks_per_cu=0, general_kernel=False, cold_passes=0):
-        """launch shape of run(); zero keeps a default (round_steps 20, cold_passes 2, interact_batch 16)"""
+    def set_launch(self, round_steps=0, interact_batch=0, blocks_per_cu=0, general_kernel=False, cold_passes=0,
+                   counter_bits=0):
+        """launch shape of run(); zero keeps a default (round_steps 20, cold_passes 2, interact_batch 16,
+        counter_bits 31)"""
         d = LaunchDesc(int(round_steps), int(cold_passes), int(interact_batch), int(blocks_per_cu),
-                       int(bool(general_kernel)))
+                       int(bool(general_kernel)), int(counter_bits))
         _check(library().ggh_set_launch(self.h, C.byref(d)))
 
     def last_grid(self):

@@ -174,6 +174,8 @@ typedef struct {
                               them to the navigation passes) */
   int32_t blocks_per_cu;   /* cap on persistent blocks per compute unit (default: as many as fit) */
   int32_t general_kernel;  /* nonzero: the general kernel variant even where the specialised one applies */
+  int32_t counter_bits;    /* the 32-bit per-lane run counters are added to the 64-bit run statistics once
+                              one reaches 2^counter_bits (1..31, default 31; lower values test the flush) */
 } ggh_launch_desc;
 int ggh_set_launch(ggh_context* ctx, const ggh_launch_desc* desc);
 /* grid and kernel variant (0 specialised, 1 general) of the last ggh_run */

@@ -141,3 +141,40 @@ def test_acquisition_script_matches_oracle(tmp_path):
         assert l1 <= 0.03 * h_ref.sum() + 20, (p, l1, h_ref.sum())
     g.delete()
     del keep
+
+
+def test_setters_drop_the_captured_acquisition():
+    """A captured acquisition holds the device addresses of the tables,
+    tallies and source.  Every setter that reallocates one of them drops the
+    graph, so a later replay reports that nothing is captured instead of
+    replaying stale addresses; capturing again works as before."""
+    from ggems_amd.engine import EngineError
+    sc = _scene()
+    eng = sc.engine_acquisition(ROT, BUDGET, 5)
+    S = sc.source
+    a = sc.arrays
+    setters = [
+        lambda: eng.set_xray_source(S["m"], S["aperture"], S["focal"], S["energy"], S["cdf"], S["is_interp"]),
+        lambda: eng.set_physics(sc.ebins, sc.cs, sc.cs_atom, a["n_elements"], a["elem_index"], a["z"],
+                                a["atom_density"], a["photon_cut"], a["density"], sc.pp, sc.factor, sc.processes),
+    ]
+    for setter in setters:
+        eng.capture_acquisition()
+        eng.replay_acquisition()
+        setter()
+        with pytest.raises(EngineError, match="no captured acquisition"):
+            eng.replay_acquisition()
+    eng.close()
+    # the dose tallies: a fresh dosimetry reallocates them
+    eng = sc.engine_acquisition(ROT, BUDGET, 5)
+    eng.capture_acquisition()
+    k = sc.engine_solids["voxel"][0]
+    pay = sc.navigators[0][2]
+    d = sc.dose_grid(pay)
+    eng.set_dosimetry(k, d["size"], d["n"], pay["bmin"], pay["bmax"], True, True, False)
+    with pytest.raises(EngineError, match="no captured acquisition"):
+        eng.replay_acquisition()
+    eng.capture_acquisition()
+    eng.replay_acquisition()
+    assert eng.stats()["primaries"] == sum(BUDGET)
+    eng.close()
