@@ -64,6 +64,11 @@ float ggo_photon_cut(const ggo_material_tables* mt, int mat, float distance, flo
 /* GGEMSAttenuations::Initialize mu / mu_en (220 bins, 1 keV..1 MeV) */
 void ggo_attenuation(const ggo_material_tables* mt, int mat, float* ebins220,
                      float* mu220, float* mu_en220);
+/* the same table with each element's energy looked up in that element's own
+ * list (the physical mixture rule; the reference searches the concatenated
+ * lists of all elements up to Z, GGEMSAttenuations.cc:200-201) */
+void ggo_attenuation_element_search(const ggo_material_tables* mt, int mat, float* ebins220,
+                                    float* mu220, float* mu_en220);
 
 /* ---- RNG (include/GGEMS/randoms/GGEMSKissEngine.hh) --------------------- */
 void ggo_mt19937_stream(uint32_t seed, uint64_t n, uint32_t* out);

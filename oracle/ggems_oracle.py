@@ -135,6 +135,8 @@ def lib():
         L.ggo_photon_cut.argtypes = [C.POINTER(MaterialTables), C.c_int, C.c_float, C.c_float]
         L.ggo_photon_cut.restype = C.c_float
         L.ggo_attenuation.argtypes = [C.POINTER(MaterialTables), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ggo_attenuation_element_search.argtypes = [C.POINTER(MaterialTables), C.c_int, C.c_void_p, C.c_void_p,
+                                                     C.c_void_p]
         L.ggo_mt19937_stream.argtypes = [C.c_uint32, C.c_uint64, C.c_void_p]
         L.ggo_seed_slots.argtypes = [C.c_uint32, C.c_uint64, C.c_void_p]
         L.ggo_kiss_uniform.argtypes = [C.c_void_p]

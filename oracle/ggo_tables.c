@@ -541,7 +541,27 @@ static int bsearch_f(float key, const float* a, int size) {
   return low;
 }
 
+/* The reference looks each energy up with BinarySearch over the
+ * concatenated per-element energy lists from index 0 up to the end of
+ * element Z's list (GGEMSAttenuations.cc:200-201).  The lists restart at
+ * 1 keV for every element, so for Z > 1 the bisection runs over an unsorted
+ * array and can land in another element's list; the resulting table is
+ * what GGEMS returns, and the oracle reproduces it (element_search = 0).
+ * element_search = 1 searches element Z's own list instead: the physical
+ * mixture rule, for the pins against the NIST tables. */
+static void attenuation_table(const ggo_material_tables* mt, int mat, int element_search, float* ebins, float* mu,
+                              float* mu_en);
+
 void ggo_attenuation(const ggo_material_tables* mt, int mat, float* ebins, float* mu, float* mu_en) {
+  attenuation_table(mt, mat, 0, ebins, mu, mu_en);
+}
+
+void ggo_attenuation_element_search(const ggo_material_tables* mt, int mat, float* ebins, float* mu, float* mu_en) {
+  attenuation_table(mt, mat, 1, ebins, mu, mu_en);
+}
+
+static void attenuation_table(const ggo_material_tables* mt, int mat, int element_search, float* ebins, float* mu,
+                              float* mu_en) {
   const int32_t* nbe = DATA(int32_t, "mu_number_of_energy_bins");
   const int32_t* idx_e = DATA(int32_t, "mu_index_energy");
   const float* data = DATA(float, "mu_data");
@@ -564,7 +584,7 @@ void ggo_attenuation(const ggo_material_tables* mt, int mat, float* ebins, float
       int z = mt->z[mt->elem_index[mat] + k];
       float frac = mt->mass_fraction[mt->elem_index[mat] + k];
       int mie = idx_e[z];
-      int ei = bsearch_f(e, energies, mie + nbe[z]);
+      int ei = element_search ? mie + bsearch_f(e, energies + mie, nbe[z]) : bsearch_f(e, energies, mie + nbe[z]);
       if (ei == mie) {
         m += mus[ei];
         men += mu_ens[ei];

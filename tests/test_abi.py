@@ -40,16 +40,59 @@ NOT_EXPORTED = {
 }
 
 
+def _reference_rows():
+    rows = []
+    for line in open(os.path.join(REPO, "tests", "golden", "reference_exports.txt")):
+        if line.strip():
+            h, n, ret, args = line.split()
+            rows.append((h, n, ret, () if args == "void" else tuple(args.split(","))))
+    return rows
+
+
+def declared_signatures(header):
+    """name -> (canonical return type, canonical argument types) of every
+    function declared in include/<header>"""
+    from signatures import parse_declaration
+    text = open(os.path.join(REPO, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", " ", text)
+    text = re.sub(r"#[^\n]*", " ", text)
+    text = re.sub(r"typedef[^;{]*(\{.*?\})?[^;]*;", " ", text, flags=re.S)
+    out = {}
+    for m in re.finditer(r"([A-Za-z_][\w\s\*]*?\b[A-Za-z_]\w*\s*\([^;{}]*\))\s*;", text):
+        name, ret, args = parse_declaration(m.group(1))
+        out[name] = (ret, args)
+    return out
+
+
 def test_ggems_header_covers_the_reference_exports():
-    """Every extern "C" function of the GGEMS headers (names listed in
+    """Every extern "C" function of the GGEMS headers (listed in
     tests/golden/reference_exports.txt) is declared in include/ggems.h and
     exported, except the headers listed in NOT_EXPORTED."""
     ours = set(declared_functions("ggems.h"))
-    rows = [l.split() for l in open(os.path.join(REPO, "tests", "golden", "reference_exports.txt")) if l.strip()]
+    rows = _reference_rows()
     assert len(rows) == 177
-    missing = [(h, n) for h, n in rows if h not in NOT_EXPORTED and n not in ours]
+    missing = [(h, n) for h, n, _, _ in rows if h not in NOT_EXPORTED and n not in ours]
     assert not missing, missing
-    assert sum(1 for h, n in rows if h not in NOT_EXPORTED) >= 118
+    assert sum(1 for h, _, _, _ in rows if h not in NOT_EXPORTED) >= 118
+
+
+def test_ggems_header_signatures_match_the_reference():
+    """Each of those functions has the reference's C signature: return type,
+    argument count, order and types, with GGEMS's typedefs resolved
+    (GGfloat = float, GGsize = size_t, ...) and class pointers as opaque
+    handles, so that GGEMS's own ctypes bindings (python_module/*.py) call
+    this library unchanged."""
+    import sys
+    sys.path.insert(0, os.path.join(REPO, "tests", "golden"))
+    ours = declared_signatures("ggems.h")
+    checked = 0
+    for h, n, ret, args in _reference_rows():
+        if h in NOT_EXPORTED:
+            continue
+        assert ours[n] == (ret, args), (h, n, ours[n], (ret, args))
+        checked += 1
+    assert checked >= 118
 
 
 def test_engine_reports_errors_without_a_device():

@@ -94,3 +94,71 @@ def test_unknown_process_raises():
         pm.add_process("Compton", "gamma", "no_such_phantom")
     del phantom
     clean_safely()
+
+
+def _reference_lookup(eb, v, e):
+    """GGEMSAttenuations::GetAttenuation (src/physics/GGEMSAttenuations.cc:269-300):
+    BinarySearch then LinearInterpolation (GGEMSMathAlgorithms.hh:46-88), in FP32."""
+    low, high = 0, eb.size - 1
+    while high - low > 1:
+        mid = (high + low) // 2
+        if eb[mid] == e:
+            low = mid
+            break
+        if eb[mid] < e:
+            low = mid
+        else:
+            high = mid
+    xa, ya, xb, yb, x = eb[low], v[low], eb[low + 1], v[low + 1], F(e)
+    if xa > xb:
+        return yb
+    if xa >= x:
+        return ya
+    if xb <= x:
+        return yb
+    return F(ya + F(F(F(x - xa) * F(yb - ya)) / F(xb - xa)))
+
+
+def test_attenuations_match_oracle_bit_for_bit():
+    """get_mu / get_mu_en_ggems_attenuations against the oracle's
+    ggo_attenuation table (pinned against NIST in test_oracle_pins.py) and
+    the reference lookup, on bin energies, between bins and at both ends of
+    the 1 keV - 1 MeV range; energies outside it raise as GGEMS does."""
+    from ggems_amd.ggems import GGEMSAttenuations
+    import ctypes as C
+    mats = ["Water", "RibBone", "GOS", "Lung", "Hydrogen"]
+    clean_safely()
+    GGEMSMaterialsDatabaseManager().set_materials(configs.MATERIALS_TXT)
+    m = GGEMSMaterials()
+    for name in mats:
+        m.add_material(name)
+    m.initialize()
+    cs = GGEMSCrossSections(m)
+    for p in PROCESSES:
+        cs.add_process(p, "gamma")
+    cs.initialize()
+    att = GGEMSAttenuations(m, cs)
+    att.initialize()
+    mt = O.material_tables(mats, cut_distance=1e-3, cut_emin=F(1e-3))
+    rng = np.random.default_rng(5)
+    for k, name in enumerate(mats):
+        eb, mu, mu_en = np.zeros(220, F), np.zeros(220, F), np.zeros(220, F)
+        O.lib().ggo_attenuation(C.byref(mt), k, O.ptr(eb), O.ptr(mu), O.ptr(mu_en))
+        energies = [F(1e-3), F(1.0)] + [eb[i] for i in range(0, 219, 7)]
+        energies += [F((eb[i] + eb[i + 1]) * F(0.5)) for i in range(3, 219, 11)]
+        energies += [F(v) for v in rng.uniform(1e-3, 1.0, 20)]
+        for e in energies:
+            got_mu = F(att.get_mu(name, float(e), "MeV"))
+            got_en = F(att.get_mu_en(name, float(e), "MeV"))
+            assert got_mu == _reference_lookup(eb, mu, e), (name, e)
+            assert got_en == _reference_lookup(eb, mu_en, e), (name, e)
+        # "keV": the energy times keV = 1e-3 in FP32 (GGEMSSystemOfUnits.hh)
+        assert F(att.get_mu(name, 60.0, "keV")) == _reference_lookup(eb, mu, F(F(60.0) * F(1e-3)))
+    for bad in (0.5e-3, 1.5):
+        with pytest.raises(GGEMSError):
+            att.get_mu("Water", bad, "MeV")
+        with pytest.raises(GGEMSError):
+            att.get_mu_en("Water", bad, "MeV")
+    att.clean()
+    cs.clean()
+    m.clean()
