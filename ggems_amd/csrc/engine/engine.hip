@@ -149,6 +149,7 @@ struct RunParams {
   const Mat34* acq_src;
   const Mat34* acq_box;
   const uint4* acq_layout;
+  const int32_t* acq_phantom;  // [n_proj] the one non-box solid projection p sees (-1: all), or null
   int32_t n_proj;
   int32_t n_box;
 };
@@ -760,8 +761,11 @@ template <bool kAcq>
 __device__ int find_and_project(const RunParams& P, const Tabs& T, f3& pos, f3 dir, uint32_t proj) {
   float dist = kOutOfWorld;
   int solid = -1;
+  // an acquisition over several phantoms: the other phantoms are absent
+  const int only = (kAcq && P.acq_phantom) ? P.acq_phantom[proj] : -1;
   for (int k = 0; k < P.n_solids; ++k) {
     const DevSolid& s = T.solids[k];
+    if (kAcq && only >= 0 && k != only && s.type != kBoxSolid) continue;
     const Mat34& m = solid_matrix<kAcq>(P, s, proj);
     f3 lp = g2l_pos(m, pos);
     if (in_aabb(lp, s.border)) {
@@ -1256,6 +1260,7 @@ struct DrrParams {
   const int32_t* w_bin;
   const float* w;
   int32_t n_w;
+  int32_t only;  // the one voxel solid present (an acquisition projection over several phantoms), or -1
   float* image;
 };
 
@@ -1333,7 +1338,7 @@ __global__ __launch_bounds__(kDrrBlock) void drr_kernel(DrrParams D) {
   const f3 ge = l2g_pos(D.box_m, pc);
   const f3 gs = l2g_pos(D.src_m, mk(0.0f, 0.0f, 0.0f));
   for (int k = 0; k < D.n_solids; ++k)
-    if (D.solids[k].type == kVoxel) drr_voxel_solid(D.solids[k], gs, ge, L);
+    if (D.solids[k].type == kVoxel && (D.only < 0 || k == D.only)) drr_voxel_solid(D.solids[k], gs, ge, L);
   double t = 0.0;
   for (int j = 0; j < D.n_w; ++j) {
     const int b = D.w_bin[j];
@@ -1531,8 +1536,10 @@ struct ggh_context {
   uint32_t acq_slots = 0;
   DevBuf<Mat34> acq_src, acq_box;
   DevBuf<uint4> acq_layout;
+  DevBuf<int32_t> acq_phantom;                      // per projection: the one non-box solid present, or -1
   std::vector<DevBuf<int32_t>> acq_hist, acq_scat;  // per solid: n_proj images (box solids)
   std::vector<Mat34> acq_src_host, acq_box_host;
+  std::vector<int32_t> acq_phantom_host;
   DevBuf<DevSolid> acq_solids;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -1673,6 +1680,8 @@ void drop_acquisition(ggh_context* c) {
   c->acq_src.release();
   c->acq_box.release();
   c->acq_layout.release();
+  c->acq_phantom.release();
+  c->acq_phantom_host.clear();
   c->acq_solids.release();
   for (auto& b : c->acq_hist) b.release();
   for (auto& b : c->acq_scat) b.release();
@@ -1748,12 +1757,18 @@ int pick_launch(ggh_context* c, uint64_t slots, bool acq, Launch& L) {
   const size_t lds = (tables_lds_bytes(c) + 15) & ~(size_t)15;
   const bool use_lds = lds <= (size_t)kLdsBudget && lds + xchg <= (size_t)c->max_lds;
   if (xchg > (size_t)c->max_lds) return fail(-3, "the block's lane-state exchange area does not fit in LDS");
-  const bool special = c->n_proc == 3 && single && use_lds && !c->ptrack.p && !c->force_general;
+  // three activated processes, tables in LDS, no fluence tracking: the
+  // specialised variant for one voxel solid, else its several-solid form
+  const bool three = c->n_proc == 3 && use_lds && !c->ptrack.p && !c->force_general;
+  const bool special = three && single;
   L.xchg_offset = use_lds ? (uint32_t)lds : 0u;
   L.shmem = L.xchg_offset + xchg;
   if (special)
     L.fn = acq ? transport_kernel<3, true, true, false, true, true>
                : transport_kernel<3, true, true, false, false, true>;
+  else if (three && regroup)
+    L.fn = acq ? transport_kernel<3, false, true, false, true, true>
+               : transport_kernel<3, false, true, false, false, true>;
   else if (acq)
     L.fn = use_lds ? (regroup ? transport_kernel<-1, false, true, true, true, true>
                               : transport_kernel<-1, false, true, true, true, false>)
@@ -1764,7 +1779,7 @@ int pick_launch(ggh_context* c, uint64_t slots, bool acq, Launch& L) {
                               : transport_kernel<-1, false, true, true, false, false>)
                    : (regroup ? transport_kernel<-1, false, false, true, false, true>
                               : transport_kernel<-1, false, false, true, false, false>);
-  c->last_variant = special ? 0 : 1;
+  c->last_variant = special ? 0 : (three && regroup ? 2 : 1);
   int per_cu = 0;
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(L.fn), kBlock,
                                                        L.shmem));
@@ -1800,6 +1815,7 @@ int acquisition_launch(ggh_context* c, RunParams& P, Launch& L) {
   P.acq_src = c->acq_src.p;
   P.acq_box = c->acq_box.p;
   P.acq_layout = c->acq_layout.p;
+  P.acq_phantom = c->acq_phantom.p;
   P.n_proj = c->acq_n_proj;
   P.n_box = n_solids_of(c, kBoxSolid);
   P.n_slots = c->acq_slots;
@@ -2376,15 +2392,24 @@ int ggh_set_acquisition(ggh_context* c, const ggh_acquisition_desc* d) {
     lay[p] = make_uint4(nb, base, rem, 0u);
     max_slots = std::max(max_slots, slots);
   }
+  std::vector<int32_t> only;
+  if (d->phantoms) {
+    only.assign(d->phantoms, d->phantoms + np);
+    for (int32_t k : only)
+      if (k < -1 || k >= (int)c->solids.size() || (k >= 0 && c->solids[k].type == kBoxSolid))
+        return fail(-2, "a projection's phantom must be -1 or the index of a voxel or meshed solid");
+  }
   std::vector<Mat34> src((size_t)np), box((size_t)np * n_box);
   for (int p = 0; p < np; ++p) src[p] = to_mat34(d->source_matrices + 16 * (size_t)p);
   for (size_t k = 0; k < box.size(); ++k) box[k] = to_mat34(d->box_matrices + 16 * k);
   drop_acquisition(c);
   c->acq_src_host = src;
   c->acq_box_host = box;
+  c->acq_phantom_host = only;
   if ((rc = dev_upload(c->acq_src, src.data(), src.size(), c->stream)) ||
       (rc = dev_upload(c->acq_box, box.data(), box.size(), c->stream)) ||
-      (rc = dev_upload(c->acq_layout, lay.data(), lay.size(), c->stream)))
+      (rc = dev_upload(c->acq_layout, lay.data(), lay.size(), c->stream)) ||
+      (!only.empty() && (rc = dev_upload(c->acq_phantom, only.data(), only.size(), c->stream))))
     return rc;
   c->acq_hist.resize(c->solids.size());
   c->acq_scat.resize(c->solids.size());
@@ -2534,6 +2559,7 @@ int drr_setup(ggh_context* c, int solid, int projection, const float* image, boo
   }
   std::memset(&D, 0, sizeof D);
   const DevSolid& box = c->solids[solid];
+  D.only = -1;
   if (projection < 0) {
     D.src_m = c->src_m;
     D.box_m = box.m;
@@ -2542,6 +2568,7 @@ int drr_setup(ggh_context* c, int solid, int projection, const float* image, boo
     if (projection >= c->acq_n_proj) return fail(-2, "no such projection");
     D.src_m = c->acq_src_host[projection];
     D.box_m = c->acq_box_host[(size_t)projection * n_box + box.box_index];
+    if (!c->acq_phantom_host.empty()) D.only = c->acq_phantom_host[projection];
   }
   D.solids = c->solids_dev.p;
   D.n_solids = (int32_t)c->solids.size();

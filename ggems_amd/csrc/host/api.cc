@@ -54,8 +54,8 @@ void run_ggems(GGEMS* g) {
   API_TRY g->run();
   API_CATCH()
 }
-void run_acquisition_ggems(GGEMS* g, size_t n, const float* rx, const float* ry, const float* rz, char const* unit,
-                           const uint64_t* n_particles) {
+void run_acquisition_phantoms_ggems(GGEMS* g, size_t n, const float* rx, const float* ry, const float* rz,
+                                    char const* unit, const uint64_t* n_particles, const int32_t* phantoms) {
   API_TRY if (!rx || !ry || !rz || !n_particles) raise("GGEMS", "null acquisition arguments");
   std::vector<float> a(3 * n);
   for (size_t p = 0; p < n; ++p) {
@@ -63,8 +63,12 @@ void run_acquisition_ggems(GGEMS* g, size_t n, const float* rx, const float* ry,
     a[n + p] = angle_unit(ry[p], str(unit));
     a[2 * n + p] = angle_unit(rz[p], str(unit));
   }
-  g->run_acquisition(n, a.data(), a.data() + n, a.data() + 2 * n, n_particles);
+  g->run_acquisition(n, a.data(), a.data() + n, a.data() + 2 * n, n_particles, phantoms);
   API_CATCH()
+}
+void run_acquisition_ggems(GGEMS* g, size_t n, const float* rx, const float* ry, const float* rz, char const* unit,
+                           const uint64_t* n_particles) {
+  run_acquisition_phantoms_ggems(g, n, rx, ry, rz, unit, n_particles, nullptr);
 }
 void repeat_acquisition_ggems(GGEMS* g) {
   API_TRY g->repeat_acquisition();

@@ -687,13 +687,25 @@ void GGEMS::initialize(uint32_t s) {
   for (GGEMSNavigator* n : navs)
     if (n->cross_sections.processes != procs)
       raise("GGEMS", "every navigator must activate the same photon processes in the same order");
-  // concatenate the navigators' tables into the engine's global material space
+  // concatenate the navigators' tables into the engine's global material
+  // space.  A navigator whose tables equal an earlier navigator's in every
+  // value (same materials in the same order, same cuts; e.g. several
+  // phantoms of one acquisition) shares that navigator's materials, which
+  // keeps the step tables small enough to stage in LDS.
   const GGEMSProcessesManager& pm = GGEMSProcessesManager::get();
   const int nb = pm.nbins;
-  int nmat = 0, nel = 0;
-  for (GGEMSNavigator* n : navs) {
-    nmat += (int)n->materials.names.size();
-    nel += n->materials.tables.total_elements();
+  auto same_tables = [](const GGEMSNavigator* a, const GGEMSNavigator* b) {
+    const MaterialTables &x = a->materials.tables, &y = b->materials.tables;
+    return x.names == y.names && x.n_elements == y.n_elements && x.density == y.density &&
+           x.photon_cut == y.photon_cut && x.z == y.z && x.atom_density == y.atom_density &&
+           a->cross_sections.tables.cs == b->cross_sections.tables.cs;
+  };
+  std::vector<int> shares(navs.size(), -1);  // the earlier navigator whose materials a navigator reuses
+  int nmat = 0;
+  for (size_t i = 0; i < navs.size(); ++i) {
+    for (size_t j = 0; j < i && shares[i] < 0; ++j)
+      if (shares[j] < 0 && same_tables(navs[i], navs[j])) shares[i] = (int)j;
+    if (shares[i] < 0) nmat += (int)navs[i]->materials.names.size();
   }
   if (nmat == 0) raise("GGEMS", "no material defined in the navigators");
   std::vector<float> cs((size_t)3 * nmat * nb, 0.0f), cs_atom((size_t)101 * nb, 0.0f), ad, cut, rho;
@@ -701,7 +713,12 @@ void GGEMS::initialize(uint32_t s) {
   std::vector<uint8_t> z;
   std::vector<int> offsets;
   int moff = 0;
-  for (GGEMSNavigator* n : navs) {
+  for (size_t i = 0; i < navs.size(); ++i) {
+    GGEMSNavigator* n = navs[i];
+    if (shares[i] >= 0) {
+      offsets.push_back(offsets[shares[i]]);
+      continue;
+    }
     offsets.push_back(moff);
     const MaterialTables& mt = n->materials.tables;
     const CrossSectionTables& ct = n->cross_sections.tables;
@@ -788,7 +805,8 @@ void GGEMS::run() {
   log(0, "GGEMS", "GGEMS simulation succeeded");
 }
 
-void GGEMS::run_acquisition(size_t n, const float* rx, const float* ry, const float* rz, const uint64_t* np) {
+void GGEMS::run_acquisition(size_t n, const float* rx, const float* ry, const float* rz, const uint64_t* np,
+                            const int32_t* phantom) {
   if (!initialized) raise("GGEMS", "GGEMS is not initialized!!!");
   auto& sources = GGEMSSourceManager::get().sources;
   if (sources.size() != 1) raise("GGEMS", "A CT acquisition needs exactly one X-ray source!!!");
@@ -809,12 +827,26 @@ void GGEMS::run_acquisition(size_t n, const float* rx, const float* ry, const fl
       std::memcpy(&box_m[16 * (p * modules.size() + j)], u.m.data(), sizeof(float) * 16);
     }
   }
+  // phantom navigator ranks -> engine solids
+  std::vector<int32_t> only;
+  if (phantom) {
+    std::vector<int> solids;
+    for (GGEMSNavigator* nav : GGEMSNavigatorManager::get().navigators)
+      if (nav->phantom_solid() >= 0) solids.push_back(nav->phantom_solid());
+    only.resize(n);
+    for (size_t p = 0; p < n; ++p) {
+      if (phantom[p] < -1 || phantom[p] >= (int32_t)solids.size())
+        raise("GGEMS", "Projection " + std::to_string(p) + " names no phantom navigator!!!");
+      only[p] = phantom[p] < 0 ? -1 : solids[phantom[p]];
+    }
+  }
   upload_source(*sources.front());
   ggh_acquisition_desc d;
   d.n_projections = (int32_t)n;
   d.source_matrices = src_m.data();
   d.box_matrices = box_m.data();
   d.n_particles = np;
+  d.phantoms = only.empty() ? nullptr : only.data();
   engine_check(ggh_set_acquisition(ctx, &d), "set acquisition");
   acquisition_projections = (int)n;
   engine_check(ggh_capture_acquisition(ctx), "capture acquisition");

@@ -68,6 +68,8 @@ struct GGEMSNavigator {
   // mesh) and OBB borders (GGEMSSolid::GetVoxelSizes / GetOBBGeometry);
   // false for navigators that take no dose map
   virtual bool dose_geometry(int&, float[3], float[3], float[3]) const { return false; }
+  // the engine solid of a phantom navigator (voxelized or meshed), else -1
+  virtual int phantom_solid() const { return -1; }
 };
 
 // GGEMSVoxelizedPhantom + GGEMSVoxelizedSolid (src/navigators/GGEMSVoxelizedPhantom.cc,
@@ -85,6 +87,7 @@ struct GGEMSVoxelizedPhantom : GGEMSNavigator {
   int register_solids(ggh_context* ctx, int material_offset) override;
   void save_results(ggh_context* ctx) override;
   bool dose_geometry(int& solid, float vs[3], float lo[3], float hi[3]) const override;
+  int phantom_solid() const override { return solid_index; }
 };
 
 // GGEMSMeshedPhantom + GGEMSMeshedSolid (src/navigators/GGEMSMeshedPhantom.cc,
@@ -101,6 +104,7 @@ struct GGEMSMeshedPhantom : GGEMSNavigator {
   int register_solids(ggh_context* ctx, int material_offset) override;
   void save_results(ggh_context* ctx) override;
   bool dose_geometry(int& solid, float vs[3], float lo[3], float hi[3]) const override;
+  int phantom_solid() const override { return solid_index; }
 };
 
 // GGEMSCTSystem + GGEMSSystem + GGEMSSolidBox (src/navigators/GGEMSCTSystem.cc,
@@ -216,7 +220,10 @@ struct GGEMS {
   // CT acquisition (one launch, captured as a hipGraph): projection p turns
   // the source and every CT system by rotation[p] (radians) on top of their
   // placement and transports n_particles[p] primaries
-  void run_acquisition(size_t n, const float* rx, const float* ry, const float* rz, const uint64_t* n_particles);
+  // phantom (optional): projection p sees only phantom navigator phantom[p]
+  // (rank among the phantom navigators in registration order; -1: all)
+  void run_acquisition(size_t n, const float* rx, const float* ry, const float* rz, const uint64_t* n_particles,
+                       const int32_t* phantom = nullptr);
   void repeat_acquisition();  // replays the captured acquisition (more primaries, same placements)
   void autotune(uint64_t n_trial);
   void save_acquisition();

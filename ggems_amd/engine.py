@@ -71,7 +71,7 @@ class LaunchDesc(C.Structure):
 
 class AcquisitionDesc(C.Structure):
     _fields_ = [("n_projections", C.c_int32), ("source_matrices", C.c_void_p), ("box_matrices", C.c_void_p),
-                ("n_particles", C.c_void_p)]
+                ("n_particles", C.c_void_p), ("phantoms", C.c_void_p)]
 
 
 class Stats(C.Structure):
@@ -407,7 +407,8 @@ class Engine:
         _check(library().ggh_set_launch(self.h, C.byref(d)))
 
     def last_grid(self):
-        """(blocks, block size, kernel variant: 0 specialised, 1 general) of the last run"""
+        """(blocks, block size, kernel variant: 0 specialised, 1 general, 2 three
+        processes over several solids) of the last run"""
         b, s, v = C.c_int32(), C.c_int32(), C.c_int32()
         _check(library().ggh_last_grid(self.h, C.byref(b), C.byref(s), C.byref(v)))
         return b.value, s.value, v.value
@@ -420,18 +421,26 @@ class Engine:
         return fast, ref
 
     # ---- CT acquisition ---------------------------------------------------
-    def set_acquisition(self, source_matrices, box_matrices, n_particles):
+    def set_acquisition(self, source_matrices, box_matrices, n_particles, phantoms=None):
         """source_matrices [P][4][4], box_matrices [P][n_box][4][4] (box solids
         in registration order), n_particles [P]: one persistent launch per
-        run_acquisition, equal to P consecutive runs with those placements."""
+        run_acquisition, equal to P consecutive runs with those placements.
+        phantoms [P] (optional): projection p sees the box solids and only
+        solid phantoms[p] among the others (-1: every solid)."""
         src = np.ascontiguousarray(source_matrices, np.float32).reshape(-1, 16)
         n_proj = src.shape[0]
         box = np.ascontiguousarray(box_matrices, np.float32).reshape(n_proj, -1)
         n = np.ascontiguousarray(n_particles, np.uint64).reshape(-1)
         if n.size != n_proj:
             raise EngineError("one photon budget per projection")
-        self._acq = (src, box, n)
-        d = AcquisitionDesc(n_proj, _ptr(src).value, _ptr(box).value if box.size else None, _ptr(n).value)
+        ph = None
+        if phantoms is not None:
+            ph = np.ascontiguousarray(phantoms, np.int32).reshape(-1)
+            if ph.size != n_proj:
+                raise EngineError("one phantom index per projection")
+        self._acq = (src, box, n, ph)
+        d = AcquisitionDesc(n_proj, _ptr(src).value, _ptr(box).value if box.size else None, _ptr(n).value,
+                            _ptr(ph).value if ph is not None else None)
         _check(library().ggh_set_acquisition(self.h, C.byref(d)))
         self.n_projections = n_proj
 

@@ -550,10 +550,12 @@ class GGEMS:
     def run(self):
         _call("run_ggems", None, [VP], self.obj)
 
-    def run_acquisition(self, rotations, n_particles, unit="deg"):
+    def run_acquisition(self, rotations, n_particles, unit="deg", phantoms=None):
         """CT acquisition in one launch: projection p turns the source and every
         CT system by rotations[p] = (rx, ry, rz) and transports n_particles[p]
-        primaries; each CT system saves a stack of projections."""
+        primaries; each CT system saves a stack of projections.  phantoms[p]
+        (optional): projection p sees only that phantom navigator (its rank
+        among the voxelized and meshed phantoms in creation order, -1 all)."""
         import numpy as np
         r = np.ascontiguousarray(rotations, np.float32).reshape(-1, 3)
         n = np.ascontiguousarray(n_particles, np.uint64).reshape(-1)
@@ -561,9 +563,17 @@ class GGEMS:
             raise GGEMSError("one photon budget per projection")
         rx, ry, rz = (np.ascontiguousarray(r[:, i]) for i in range(3))
         fp = C.POINTER(C.c_float)
-        _call("run_acquisition_ggems", None, [VP, C.c_size_t, fp, fp, fp, C.c_char_p, C.POINTER(C.c_uint64)],
-              self.obj, r.shape[0], rx.ctypes.data_as(fp), ry.ctypes.data_as(fp), rz.ctypes.data_as(fp),
-              unit.encode(), n.ctypes.data_as(C.POINTER(C.c_uint64)))
+        args = [self.obj, r.shape[0], rx.ctypes.data_as(fp), ry.ctypes.data_as(fp), rz.ctypes.data_as(fp),
+                unit.encode(), n.ctypes.data_as(C.POINTER(C.c_uint64))]
+        types = [VP, C.c_size_t, fp, fp, fp, C.c_char_p, C.POINTER(C.c_uint64)]
+        if phantoms is None:
+            _call("run_acquisition_ggems", None, types, *args)
+            return
+        ph = np.ascontiguousarray(phantoms, np.int32).reshape(-1)
+        if ph.size != r.shape[0]:
+            raise GGEMSError("one phantom per projection")
+        _call("run_acquisition_phantoms_ggems", None, types + [C.POINTER(C.c_int32)], *args,
+              ph.ctypes.data_as(C.POINTER(C.c_int32)))
 
     def repeat_acquisition(self):
         _call("repeat_acquisition_ggems", None, [VP], self.obj)

@@ -273,6 +273,12 @@ void draw_sphere(GGEMSSphere* sphere);
  * equal those of n consecutive runs with those placements. */
 void run_acquisition_ggems(GGEMS* ggems, size_t n, const float* rx, const float* ry, const float* rz,
                            char const* unit, const uint64_t* n_particles);
+/* the same over several phantoms in one launch: projection p sees the CT
+ * systems and only phantom navigator phantoms[p] (its rank among the
+ * voxelized and meshed phantoms in creation order; -1: every phantom).
+ * Equal to the consecutive runs of a world holding that one phantom. */
+void run_acquisition_phantoms_ggems(GGEMS* ggems, size_t n, const float* rx, const float* ry, const float* rz,
+                                    char const* unit, const uint64_t* n_particles, const int32_t* phantoms);
 /* replays the last acquisition (its random streams continue) and saves the accumulated stacks */
 void repeat_acquisition_ggems(GGEMS* ggems);
 /* times trial runs of n_trial primaries over launch shapes and keeps the

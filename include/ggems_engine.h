@@ -178,7 +178,8 @@ typedef struct {
                               one reaches 2^counter_bits (1..31, default 31; lower values test the flush) */
 } ggh_launch_desc;
 int ggh_set_launch(ggh_context* ctx, const ggh_launch_desc* desc);
-/* grid and kernel variant (0 specialised, 1 general) of the last ggh_run */
+/* grid and kernel variant of the last ggh_run: 0 specialised (three processes,
+ * one voxel solid), 1 general, 2 three processes over several solids */
 int ggh_last_grid(ggh_context* ctx, int32_t* blocks, int32_t* block_size, int32_t* variant);
 
 int ggh_get_histogram(ggh_context* ctx, int solid, int32_t* counts, int32_t* scatter);
@@ -194,13 +195,17 @@ int ggh_get_histogram(ggh_context* ctx, int solid, int32_t* counts, int32_t* sca
  * primaries of projection 0, then 1, ..., so its random stream continues
  * exactly as over consecutive ggh_run calls: the images equal those of
  * n_projections ggh_run calls with the same placements.  Each box solid
- * scores n_projections images; phantom dose tallies sum over projections. */
+ * scores n_projections images; phantom dose tallies sum over projections.
+ * Several phantoms share one launch through `phantoms`: projection p then
+ * sees the box solids and only the one other solid phantoms[p] (-1: every
+ * solid), as if the other phantoms had not been registered for that run. */
 #define GGH_MAX_PROJECTIONS 65536
 typedef struct {
   int32_t n_projections;
   const float* source_matrices;  /* [n_projections][16] row-major local->global */
   const float* box_matrices;     /* [n_projections][n_box_solids][16] */
   const uint64_t* n_particles;   /* [n_projections] */
+  const int32_t* phantoms;       /* [n_projections] solid index or -1; NULL: every solid in every projection */
 } ggh_acquisition_desc;
 /* uploads placements and zeroes the images (tallies of earlier acquisitions are dropped) */
 int ggh_set_acquisition(ggh_context* ctx, const ggh_acquisition_desc* desc);

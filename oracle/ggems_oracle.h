@@ -179,10 +179,13 @@ uint64_t ggo_run_slots(const ggo_physics* ph, const ggo_solid* solids, int n_sol
  * the reference's batch layout; slot s runs its primaries of projection 0,
  * 1, ... in order, continuing its JKISS state as consecutive GGEMS::Run
  * calls would (src/global/GGEMS.cc:303-363).  The histogram/scatter of
- * every box solid points at n_projections consecutive images. */
+ * every box solid points at n_projections consecutive images.  phantoms
+ * (NULL, or one entry per projection): projection p runs in a world of the
+ * box solids and only solid phantoms[p] of the others (-1: every solid). */
 int ggo_run_acquisition(const ggo_physics* ph, const ggo_solid* solids, int n_solids, const ggo_source* src,
                         int n_projections, const float* src_mats, const float* box_mats,
-                        const uint64_t* n_particles, uint32_t seed, int threads, ggo_stats* stats);
+                        const uint64_t* n_particles, const int32_t* phantoms, uint32_t seed, int threads,
+                        ggo_stats* stats);
 
 /* number of batches / slots for n particles (reference batch split) */
 void ggo_batch_layout(uint64_t n_particles, uint64_t* n_batches, uint64_t* n_slots);

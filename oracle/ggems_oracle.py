@@ -149,7 +149,8 @@ def lib():
                                     C.c_uint32, C.c_uint64, C.c_uint64, C.c_int, C.POINTER(Stats)]
         L.ggo_run_slots.restype = C.c_uint64
         L.ggo_run_acquisition.argtypes = [C.POINTER(Physics), C.POINTER(Solid), C.c_int, C.POINTER(Source), C.c_int,
-                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.POINTER(Stats)]
+                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int,
+                                          C.POINTER(Stats)]
         L.ggo_batch_layout.argtypes = [C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         L.ggo_voxel_walk.argtypes = [C.POINTER(Solid), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
         L.ggo_sample_compton.argtypes = [C.c_float, C.c_void_p, C.c_void_p]

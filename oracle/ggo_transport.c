@@ -863,6 +863,7 @@ typedef struct {
   int np;
   const float* src_mats;
   const float* box_mats;
+  const int32_t* phantoms; /* [np] the one non-box solid present, -1 all; or NULL */
   int n_box;
   const uint64_t* lay; /* [np][3]: n_batches, base, rem */
   uint32_t* states;
@@ -875,7 +876,12 @@ static void* acq_run(void* arg) {
   acq_worker* w = (acq_worker*)arg;
   memset(&w->st, 0, sizeof w->st);
   ggo_solid* sol = (ggo_solid*)malloc((size_t)w->ns * sizeof(ggo_solid));
-  if (!sol) return NULL;
+  ggo_solid* present = (ggo_solid*)malloc((size_t)w->ns * sizeof(ggo_solid));
+  if (!sol || !present) {
+    free(sol);
+    free(present);
+    return NULL;
+  }
   memcpy(sol, w->solids, (size_t)w->ns * sizeof(ggo_solid));
   ggo_source src = *w->src;
   for (uint64_t s = w->s0; s < w->s1; ++s) {
@@ -895,18 +901,33 @@ static void* acq_run(void* arg) {
         sol[k].scatter = w->solids[k].scatter ? w->solids[k].scatter + img : NULL;
         ++j;
       }
-      while (todo--) transport_primary(w->ph, sol, w->ns, &src, rng, &w->st);
+      /* several phantoms in one acquisition: projection p is a run in a
+       * world of the box solids and its one phantom, in registration order */
+      const int only = w->phantoms ? w->phantoms[p] : -1;
+      if (only < 0) {
+        while (todo--) transport_primary(w->ph, sol, w->ns, &src, rng, &w->st);
+      } else {
+        int n = 0;
+        for (int k = 0; k < w->ns; ++k)
+          if (k == only || sol[k].type == GGO_SOLID_BOX) present[n++] = sol[k];
+        while (todo--) transport_primary(w->ph, present, n, &src, rng, &w->st);
+      }
     }
     for (int k = 0; k < 5; ++k) w->states[k * w->n_slots + s] = rng[k];
   }
   free(sol);
+  free(present);
   return NULL;
 }
 
 int ggo_run_acquisition(const ggo_physics* ph, const ggo_solid* solids, int ns, const ggo_source* src, int np,
-                        const float* src_mats, const float* box_mats, const uint64_t* n_particles, uint32_t seed,
-                        int threads, ggo_stats* stats) {
+                        const float* src_mats, const float* box_mats, const uint64_t* n_particles,
+                        const int32_t* phantoms, uint32_t seed, int threads, ggo_stats* stats) {
   if (rayleigh_init() || np < 1) return -1;
+  if (phantoms)
+    for (int p = 0; p < np; ++p)
+      if (phantoms[p] < -1 || phantoms[p] >= ns || (phantoms[p] >= 0 && solids[phantoms[p]].type == GGO_SOLID_BOX))
+        return -1;
   uint64_t* lay = (uint64_t*)malloc(3 * (size_t)np * sizeof(uint64_t));
   uint32_t* states = (uint32_t*)malloc(5 * MAX_SLOTS * sizeof(uint32_t));
   if (!lay || !states) {
@@ -939,6 +960,7 @@ int ggo_run_acquisition(const ggo_physics* ph, const ggo_solid* solids, int ns, 
     w->np = np;
     w->src_mats = src_mats;
     w->box_mats = box_mats;
+    w->phantoms = phantoms;
     w->n_box = n_box;
     w->lay = lay;
     w->states = states;

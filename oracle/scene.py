@@ -185,11 +185,18 @@ class Scene:
     # ---- tables ---------------------------------------------------------------
     def build(self):
         self.ebins = O.energy_bins(self.nbins, self.emin, self.emax)
+        # a navigator with the material list of an earlier one shares its
+        # materials (every navigator here has the same cut distance, so the
+        # tables are equal value for value)
         mats = []
         self.offsets = []
+        first = {}
         for kind, materials, _ in self.navigators:
-            self.offsets.append(len(mats))
-            mats.extend(materials)
+            key = tuple(materials)
+            if key not in first:
+                first[key] = len(mats)
+                mats.extend(materials)
+            self.offsets.append(first[key])
         self.materials = mats
         self.mt = O.material_tables(mats, cut_distance=self.cut_distance, cut_emin=self.emin)
         self.cs, self.cs_atom = O.cs_tables(self.mt, self.nbins, self.ebins, self.processes)
@@ -335,29 +342,32 @@ class Scene:
             box.append([mat_mult(R, mod["m"]) for mod in mods])
         return np.array(src, F).reshape(len(rotations), 4, 4), np.array(box, F).reshape(len(rotations), len(mods), 4, 4)
 
-    def run_oracle_acquisition(self, rotations, n_particles, seed, threads=0):
+    def run_oracle_acquisition(self, rotations, n_particles, seed, threads=0, phantoms=None):
         """The projections as consecutive runs (slot-major, continuing JKISS
-        states); hist/scatter are per-module stacks [P][rows][cols]."""
+        states); hist/scatter are per-module stacks [P][rows][cols].
+        phantoms[p] (optional): the one phantom solid projection p sees."""
         src, box = self.acquisition_matrices(rotations)
         tallies = {}
         ph, solids, source = self._oracle_structs(tallies, images=len(rotations))
         n = np.ascontiguousarray(n_particles, np.uint64)
+        only = None if phantoms is None else np.ascontiguousarray(phantoms, np.int32)
         st = O.Stats()
         rc = O.lib().ggo_run_acquisition(C.byref(ph), solids, len(solids), C.byref(source), len(rotations),
                                          O.ptr(np.ascontiguousarray(src)), O.ptr(np.ascontiguousarray(box)),
-                                         O.ptr(n), seed, threads, C.byref(st))
+                                         O.ptr(n), None if only is None else O.ptr(only), seed, threads,
+                                         C.byref(st))
         if rc != 0:
             raise RuntimeError("oracle acquisition failed")
         tallies["stats"] = {k: int(getattr(st, k)) for k, _ in O.Stats._fields_}
         return tallies
 
-    def engine_acquisition(self, rotations, n_particles, seed, eng=None):
+    def engine_acquisition(self, rotations, n_particles, seed, eng=None, phantoms=None):
         """An engine with the acquisition set (seeded, tallies zero)."""
         eng = eng or self.make_engine()
         src, box = self.acquisition_matrices(rotations)
         eng.seed(seed)
         eng.reset_tallies()
-        eng.set_acquisition(src, box, n_particles)
+        eng.set_acquisition(src, box, n_particles, phantoms)
         return eng
 
     def read_acquisition(self, eng):

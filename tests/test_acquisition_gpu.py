@@ -178,3 +178,153 @@ def test_setters_drop_the_captured_acquisition():
     eng.replay_acquisition()
     assert eng.stats()["primaries"] == sum(BUDGET)
     eng.close()
+
+
+# ---- several phantoms in one launch ----------------------------------------
+PH_ROT = [(0.0, 0.0, 0.1 * p) for p in range(6)]
+PH_BUDGET = [120_000, 80_000, 0, 150_000, 60_000, 100_000]
+PH_ONLY = [0, 1, 1, 2, 0, -1]  # solid indices; -1: every phantom at once
+
+
+def _phantoms_scene():
+    """three different phantoms at the isocentre, two of them with the same
+    material list (they share their materials), and one flat panel"""
+    from scene import cylinder_phantom
+    sc = Scene()
+    sc.add_phantom(cylinder_phantom(40, radius=12.0), ["Water", "RibBone"], (1.0, 1.0, 1.0))
+    sc.add_phantom(cylinder_phantom(30, radius=5.0), ["Water", "RibBone"], (1.5, 1.5, 1.5),
+                   rotation=(0.0, 0.4, 0.0), dosimetry={})
+    sc.add_phantom(np.zeros((16, 16, 16), np.uint8), ["Lung"], (2.0, 2.0, 2.0))
+    sc.add_ct_flat("GOS", (1, 1), (48, 48, 1), (1.0, 1.0, 1.0), 600.0, 1000.0, 10e-3)
+    sc.set_source((-600.0, 0.0, 0.0), 2.0, [0.06], mono=True)
+    return sc.build()
+
+
+def _l1(a, b):
+    return int(np.abs(a.astype(np.int64) - b).sum())
+
+
+def test_phantom_acquisition_matches_oracle():
+    """Projections over different phantoms in one launch against the oracle's
+    consecutive runs, history by history: each projection sees the panel and
+    its one phantom."""
+    sc = _phantoms_scene()
+    assert sc.offsets[0] == sc.offsets[1]  # the shared material list
+    eng = sc.engine_acquisition(PH_ROT, PH_BUDGET, 71, phantoms=PH_ONLY)
+    eng.run_acquisition()
+    assert eng.last_grid()[2] == 2  # three processes over several voxel solids
+    got = sc.read_acquisition(eng)
+    ref = sc.run_oracle_acquisition(PH_ROT, PH_BUDGET, 71, phantoms=PH_ONLY)
+    assert got["stats"]["primaries"] == ref["stats"]["primaries"] == sum(PH_BUDGET)
+    for k in ("voxel_steps", "interactions", "hops"):
+        assert abs(got["stats"][k] - ref["stats"][k]) <= 1e-3 * ref["stats"][k] + 10, k
+    for p in range(len(PH_ROT)):
+        h, r = got["hist"][0][p], ref["hist"][0][p]
+        l1 = _l1(h, r)
+        print("[acq] projection %d phantom %d: %d counts, L1 %d" % (p, PH_ONLY[p], int(r.sum()), l1), flush=True)
+        assert l1 <= 2e-3 * r.sum() + 4, (p, l1, int(r.sum()))
+    # the dose phantom (solid 1) is crossed only in projections 1 and 5
+    edep, _, hit, _ = eng.dose_tallies()
+    assert ref["edep"].sum() > 0
+    assert abs(edep.sum() - ref["edep"].sum()) <= 2e-3 * ref["edep"].sum()
+    assert _l1(hit, ref["hit"]) <= 2e-3 * ref["hit"].sum() + 4
+    eng.close()
+
+
+def test_phantom_acquisition_equals_consecutive_launches():
+    """One launch over projections of three phantoms == one launch per
+    projection with the streams continuing, bit for bit; the captured graph
+    replays it exactly."""
+    sc = _phantoms_scene()
+    eng = sc.engine_acquisition(PH_ROT, PH_BUDGET, 73, phantoms=PH_ONLY)
+    eng.run_acquisition()
+    one = sc.read_acquisition(eng)
+    step = sc.make_engine()
+    step.seed(73)
+    step.reset_tallies()
+    src, box = sc.acquisition_matrices(PH_ROT)
+    for p in range(len(PH_ROT)):
+        step.set_acquisition(src[p:p + 1], box[p:p + 1], PH_BUDGET[p:p + 1], PH_ONLY[p:p + 1])
+        step.run_acquisition()
+        h, s = step.projection(sc.engine_solids["box"][0], 0)
+        assert np.array_equal(h, one["hist"][0][p]), p
+        assert np.array_equal(s, one["scatter"][0][p]), p
+    assert step.stats() == one["stats"]
+    e1, _, hit1, _ = eng.dose_tallies()
+    e2, _, hit2, _ = step.dose_tallies()
+    assert np.array_equal(hit1, hit2)
+    np.testing.assert_allclose(e1, e2, rtol=1e-12, atol=0.0)
+    graph = sc.engine_acquisition(PH_ROT, PH_BUDGET, 73, phantoms=PH_ONLY)
+    graph.capture_acquisition()
+    graph.replay_acquisition()
+    g = sc.read_acquisition(graph)
+    assert np.array_equal(np.stack(g["hist"]), np.stack(one["hist"])) and g["stats"] == one["stats"]
+    for e in (eng, step, graph):
+        e.close()
+
+
+def test_phantom_acquisition_rejects_a_panel_as_phantom():
+    from ggems_amd.engine import EngineError
+    sc = _phantoms_scene()
+    eng = sc.make_engine()
+    panel = sc.engine_solids["box"][0]
+    src, box = sc.acquisition_matrices(PH_ROT[:2])
+    for bad in ([panel, 0], [7, 0], [-2, 0]):
+        with pytest.raises(EngineError):
+            eng.set_acquisition(src, box, [10, 10], bad)
+    eng.close()
+
+
+def _stated_acquisition(budget_range, seed=7):
+    """The batched CT acquisition at its stated shape: seeded ellipsoid
+    phantoms of 128^3, 256^3 and 384^3 1 mm voxels, 360 one-degree
+    projections each, the 120 kVp spectrum and the flat 512x384 GOS panel
+    of 0.78 mm pixels at SDD 1000 mm; per-projection budgets drawn uniformly
+    in budget_range."""
+    sc = Scene()
+    for k, side in enumerate((128, 256, 384)):
+        lab = authoring.ellipsoid_body_phantom(1 + k, shape=(side, side, side), voxel=1.0, n_ellipsoids=40,
+                                               body_axes=(0.45 * side, 0.35 * side),
+                                               semi_axis_range=(side / 50.0, side / 4.0))
+        sc.add_phantom(lab, authoring.BODY_MATERIALS, (1.0, 1.0, 1.0))
+    sc.add_ct_flat("GOS", (1, 1), (384, 512, 1), (0.78, 0.78, 1.0), 600.0, 1000.0, 10e-3)
+    e, w = authoring.kramers_spectrum(120.0, 20.0, 2.5, authoring.aluminium_mu())
+    half_diag = 0.5 * 0.78 * math.hypot(384, 512)
+    sc.set_source((-600.0, 0.0, 0.0), math.degrees(math.atan(half_diag / 1000.0)), e, intensities=w)
+    sc.build()
+    rot, only, budgets = [], [], []
+    for k in range(3):
+        r, b = configs.acquisition_plan(n_angles=360, budget_range=budget_range, seed=seed + k)
+        rot.append(np.radians(r.astype(np.float64)).astype(np.float32))
+        budgets.append(b)
+        only += [k] * 360
+    return sc, np.concatenate(rot), np.concatenate(budgets), np.array(only, np.int32)
+
+
+def test_stated_acquisition_in_one_launch_matches_oracle():
+    """The three stated phantoms x 360 projections (1,080 projections of
+    uneven budgets, here 10^4 to 3*10^4 primaries each) in ONE captured
+    launch against the oracle's consecutive runs on the same inputs, history
+    by history: per-projection counts and the L1 distance of the images."""
+    sc, rot, budgets, only = _stated_acquisition((10_000, 30_000))
+    eng = sc.engine_acquisition(rot, budgets, 19, phantoms=only)
+    eng.capture_acquisition()
+    eng.replay_acquisition()
+    assert eng.last_grid()[2] == 2
+    got = sc.read_acquisition(eng)
+    ref = sc.run_oracle_acquisition(rot, budgets, 19, threads=0, phantoms=only)
+    n = int(budgets.sum())
+    assert got["stats"]["primaries"] == ref["stats"]["primaries"] == n
+    for k in ("voxel_steps", "interactions", "hops"):
+        assert abs(got["stats"][k] - ref["stats"][k]) <= 1e-3 * ref["stats"][k], k
+    h, r = got["hist"][0], ref["hist"][0]
+    per = np.abs(h.astype(np.int64) - r).reshape(len(budgets), -1).sum(axis=1)
+    counts = r.reshape(len(budgets), -1).sum(axis=1)
+    tot = int(counts.sum())
+    print("[acq] stated acquisition: %d primaries, %d counts, L1 %d (%.2e of the counts), worst projection "
+          "L1 %d of %d counts; kernel %.1f ms" % (n, tot, int(per.sum()), per.sum() / tot, int(per.max()),
+                                                 int(counts[per.argmax()]), got["kernel_ms"]), flush=True)
+    assert (counts > 0).all()
+    assert per.sum() <= 2e-3 * tot
+    assert (per <= 0.01 * counts + 6).all()
+    eng.close()

@@ -6,6 +6,7 @@ import copy
 import math
 
 import numpy as np
+import pytest
 
 from scene import Scene, Transform, cfg1_scene, mat_mult
 
@@ -77,3 +78,48 @@ def test_cylinder_projections_agree_under_rotation():
     pa, pb = a.sum(axis=0), b.sum(axis=0)
     chi2 = float((((pa - pb) ** 2) / np.maximum(pa + pb, 1)).sum())
     assert chi2 < len(pa) + 6 * math.sqrt(2 * len(pa)), chi2
+
+
+def _two_phantom_scene(keep=(0, 1)):
+    """two different phantoms at the isocentre (a water cube and a bone-cored
+    cylinder), of which `keep` are registered, and one flat panel"""
+    from scene import cylinder_phantom
+    sc = Scene()
+    if 0 in keep:
+        sc.add_phantom(np.zeros((12, 12, 12), np.uint8), ["Water"], (1.5, 1.5, 1.5))
+    if 1 in keep:
+        sc.add_phantom(cylinder_phantom(24, radius=6.0), ["Water", "RibBone"], (1.0, 1.0, 1.0),
+                       rotation=(0.0, 0.3, 0.0), dosimetry={})
+    sc.add_ct_flat("GOS", (1, 1), (20, 20, 1), (1.0, 1.0, 1.0), 600.0, 1000.0, 10e-3)
+    sc.set_source((-600.0, 0.0, 0.0), 0.6, [0.06], mono=True)
+    return sc.build()
+
+
+def test_projection_sees_only_its_phantom():
+    """A projection over phantom solid k of a world holding both phantoms is
+    the plain acquisition of the world holding phantom k alone: the same
+    histories, bit for bit, including the dose map of the phantom kept."""
+    both = _two_phantom_scene()
+    rot = [(0.0, 0.0, 0.0), (0.0, 0.0, 0.9)]
+    n = [60_000, 90_000]
+    for k in (0, 1):
+        acq = both.run_oracle_acquisition(rot, n, 21, threads=8, phantoms=[k, k])
+        alone = _two_phantom_scene(keep=(k,)).run_oracle_acquisition(rot, n, 21, threads=8)
+        assert np.array_equal(acq["hist"][0], alone["hist"][0]), k
+        assert np.array_equal(acq["scatter"][0], alone["scatter"][0]), k
+        assert acq["stats"] == alone["stats"], k
+        if k == 1:
+            assert alone["edep"].sum() > 0 and np.array_equal(acq["edep"], alone["edep"])
+        else:
+            assert not acq["edep"].any()  # the dose phantom is absent from every projection
+    # -1 keeps every solid: the plain acquisition of the world with both
+    acq = both.run_oracle_acquisition(rot, n, 21, threads=8, phantoms=[-1, -1])
+    plain = both.run_oracle_acquisition(rot, n, 21, threads=8)
+    assert np.array_equal(acq["hist"][0], plain["hist"][0])
+
+
+def test_projection_phantom_must_name_a_phantom():
+    both = _two_phantom_scene()
+    for bad in ([2, 0], [5, 0], [-2, 0]):  # solid 2 is the panel
+        with pytest.raises(RuntimeError):
+            both.run_oracle_acquisition([(0.0, 0.0, 0.0)] * 2, [10, 10], 1, threads=2, phantoms=bad)
