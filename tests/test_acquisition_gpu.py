@@ -9,6 +9,7 @@ import os
 import numpy as np
 import pytest
 
+import parity
 from ggems_amd import authoring, configs
 from scene import Scene, cfg1_scene
 
@@ -36,12 +37,10 @@ def test_acquisition_matches_oracle():
     assert stats["primaries"] == ref["stats"]["primaries"] == sum(BUDGET)
     assert not hist[1].any()
     for p in (0, 2, 3):
-        h_ref = ref["hist"][0][p]
-        l1 = np.abs(hist[p].astype(np.int64) - h_ref).sum()
-        assert l1 <= 0.02 * h_ref.sum() + 20, (p, l1, h_ref.sum())
-    assert abs(stats["voxel_steps"] - ref["stats"]["voxel_steps"]) <= 0.01 * ref["stats"]["voxel_steps"]
+        parity.l1("acquisition projection %d" % p, ref["hist"][0][p], hist[p], 2e-3, 10)
+    parity.counters("acquisition", ref["stats"], stats, ("voxel_steps",), 1e-4, 0)
     edep, _, _, _ = eng.dose_tallies()
-    assert abs(edep.sum() - ref["edep"].sum()) <= 0.01 * ref["edep"].sum()
+    parity.total("acquisition edep", ref["edep"], edep, 1e-4)
     eng.close()
 
 
@@ -136,9 +135,7 @@ def test_acquisition_script_matches_oracle(tmp_path):
     sc.build()
     ref = sc.run_oracle_acquisition(np.radians(rot.astype(np.float64)).astype(np.float32), budgets, 123)
     for p in range(3):
-        h_ref = ref["hist"][0][p]
-        l1 = np.abs(stack[p].astype(np.int64) - h_ref).sum()
-        assert l1 <= 0.03 * h_ref.sum() + 20, (p, l1, h_ref.sum())
+        parity.l1("acquisition script projection %d" % p, ref["hist"][0][p], stack[p], 1e-3, 10)
     g.delete()
     del keep
 
@@ -301,11 +298,35 @@ def _stated_acquisition(budget_range, seed=7):
     return sc, np.concatenate(rot), np.concatenate(budgets), np.array(only, np.int32)
 
 
+def _binned_chi2(a, b, block):
+    """Two-sample chi-square of two count images summed over block x block
+    pixels: sum (a - b)^2 / (a + b) over bins with counts, and its degrees
+    of freedom."""
+    rows, cols = a.shape[-2] // block * block, a.shape[-1] // block * block
+    def binned(x):
+        x = x[..., :rows, :cols].astype(np.float64)
+        return x.reshape(-1, rows // block, block, cols // block, block).sum(axis=(0, 2, 4))
+    A, B = binned(a), binned(b)
+    m = (A + B) > 0
+    return float(((A - B)[m] ** 2 / (A + B)[m]).sum()), int(m.sum())
+
+
 def test_stated_acquisition_in_one_launch_matches_oracle():
     """The three stated phantoms x 360 projections (1,080 projections of
     uneven budgets, here 10^4 to 3*10^4 primaries each) in ONE captured
-    launch against the oracle's consecutive runs on the same inputs, history
-    by history: per-projection counts and the L1 distance of the images."""
+    launch.
+
+    A particle slot runs its photons of projection 0, then 1, ... 1,079 in
+    sequence, so once an ulp of a device transcendental changes one history
+    of a slot, that slot's later histories differ from the oracle's as well:
+    the images of the first projections agree history by history, and later
+    ones only statistically.  So the launch is checked three ways:
+    - the first ten projections against the oracle's, image by image;
+    - the whole stack against the oracle by a two-sample chi-square per
+      phantom (its 360 images summed, 16 x 16-pixel bins) and per projection
+      (counts), plus the run counters;
+    - bit for bit against three consecutive launches, one per phantom, with
+      the random streams continuing."""
     sc, rot, budgets, only = _stated_acquisition((10_000, 30_000))
     eng = sc.engine_acquisition(rot, budgets, 19, phantoms=only)
     eng.capture_acquisition()
@@ -320,11 +341,36 @@ def test_stated_acquisition_in_one_launch_matches_oracle():
     h, r = got["hist"][0], ref["hist"][0]
     per = np.abs(h.astype(np.int64) - r).reshape(len(budgets), -1).sum(axis=1)
     counts = r.reshape(len(budgets), -1).sum(axis=1)
+    mine = h.reshape(len(budgets), -1).sum(axis=1)
     tot = int(counts.sum())
-    print("[acq] stated acquisition: %d primaries, %d counts, L1 %d (%.2e of the counts), worst projection "
-          "L1 %d of %d counts; kernel %.1f ms" % (n, tot, int(per.sum()), per.sum() / tot, int(per.max()),
-                                                 int(counts[per.argmax()]), got["kernel_ms"]), flush=True)
     assert (counts > 0).all()
-    assert per.sum() <= 2e-3 * tot
-    assert (per <= 0.01 * counts + 6).all()
+    head = int(per[:10].sum()), int(counts[:10].sum())
+    print("[acq] stated acquisition: %d primaries, %d counts, kernel %.1f ms; first 10 projections L1 %d of %d "
+          "counts; whole stack L1 %d (%.2e of the counts)" % (n, tot, got["kernel_ms"], head[0], head[1],
+                                                              int(per.sum()), per.sum() / tot), flush=True)
+    assert head[0] <= 5e-3 * head[1] + 4
+    for k in range(3):
+        sel = only == k
+        chi2, dof = _binned_chi2(h[sel], r[sel], 16)
+        pc = mine[sel].astype(np.float64), counts[sel].astype(np.float64)
+        chi2_p = float(((pc[0] - pc[1]) ** 2 / (pc[0] + pc[1])).sum())
+        print("[acq] phantom %d: image chi2/dof %.3f (dof %d), per-projection counts chi2/dof %.3f (dof %d)"
+              % (k, chi2 / dof, dof, chi2_p / sel.sum(), int(sel.sum())), flush=True)
+        assert chi2 / dof < 1.3 and chi2_p / sel.sum() < 1.3
+    # the same acquisition as three launches, one phantom each
+    step = sc.engine_acquisition(rot[:360], budgets[:360], 19, phantoms=only[:360])
+    src, box = sc.acquisition_matrices(rot)
+    panel = sc.engine_solids["box"][0]
+    for k in range(3):
+        sel = slice(360 * k, 360 * (k + 1))
+        if k:
+            step.set_acquisition(src[sel], box[sel], budgets[sel], only[sel])
+        step.run_acquisition()
+        for p in range(360):
+            hp, sp = step.projection(panel, p)
+            assert np.array_equal(hp, h[360 * k + p]), (k, p)
+            if got["scatter"]:
+                assert np.array_equal(sp, got["scatter"][0][360 * k + p]), (k, p)
+    assert step.stats() == got["stats"]
     eng.close()
+    step.close()

@@ -16,6 +16,7 @@ import os
 import numpy as np
 import pytest
 
+import parity
 from scene import REPO, voxel_ct_scene
 
 pytestmark = pytest.mark.gpu
@@ -35,15 +36,14 @@ def test_full_phantom_run_matches_oracle(body):
     rs, gs = ref["stats"], got["stats"]
     assert gs["primaries"] == rs["primaries"] == n
     assert rs["voxel_steps"] > 100 * n  # the photons cross the body
-    for k in ("voxel_steps", "interactions", "detector_interactions", "hops"):
-        assert abs(gs[k] - rs[k]) <= 0.01 * rs[k] + 10, (k, gs[k], rs[k])
+    parity.counters("full phantom", rs, gs, ("voxel_steps", "interactions", "detector_interactions", "hops"), 2e-3)
     h_ref, h_got = ref["hist"][0], got["hist"][0]
     assert h_ref.sum() > 100
     # histories differ only where an ulp of a device transcendental flipped one
-    assert np.abs(h_ref.astype(np.int64) - h_got).sum() <= 0.02 * h_ref.sum() + 10
-    assert abs(got["edep"].sum() - ref["edep"].sum()) <= 0.01 * ref["edep"].sum()
-    assert abs(got["edep2"].sum() - ref["edep2"].sum()) <= 0.02 * ref["edep2"].sum()
-    assert np.abs(ref["hit"].astype(np.int64) - got["hit"]).sum() <= 0.02 * ref["hit"].sum()
+    parity.l1("full phantom histogram", h_ref, h_got, 5e-3, 10)
+    parity.total("full phantom edep", ref["edep"], got["edep"], 1e-3)
+    parity.total("full phantom edep2", ref["edep2"], got["edep2"], 1e-3)
+    parity.l1("full phantom hit", ref["hit"], got["hit"], 1e-2)
 
 
 def test_full_step_launch_invariance_and_oracle_sample(body):

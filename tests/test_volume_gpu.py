@@ -9,6 +9,7 @@ import os
 import numpy as np
 import pytest
 
+import parity
 import volume as V
 from ggems_amd import authoring, configs
 from ggems_amd.ggems import (GGEMS, GGEMSBox, GGEMSSphere, GGEMSTube, GGEMSVolumeCreatorManager,
@@ -106,8 +107,7 @@ def test_tube_phantom_simulation_matches_oracle(tmp_path):
     sc.build()
     ref = sc.run_oracle(200_000, 55)
     h_ref = ref["hist"][0]
-    l1 = np.abs(proj[0].astype(np.int64) - h_ref).sum()
     assert h_ref.sum() > 1000
-    assert l1 <= 0.03 * h_ref.sum() + 20, (l1, h_ref.sum())
+    parity.l1("tube phantom projection", h_ref, proj[0], 1e-3, 10)
     g.delete()
     del phantom, ct, src
