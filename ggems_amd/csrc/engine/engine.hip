@@ -35,6 +35,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -1260,6 +1261,7 @@ struct DrrParams {
   const int32_t* w_bin;
   const float* w;
   int32_t n_w;
+  const uint32_t* n_w_dev;  // weight count on the device (the allocation-free entry point), else null
   int32_t only;  // the one voxel solid present (an acquisition projection over several phantoms), or -1
   float* image;
 };
@@ -1340,7 +1342,8 @@ __global__ __launch_bounds__(kDrrBlock) void drr_kernel(DrrParams D) {
   for (int k = 0; k < D.n_solids; ++k)
     if (D.solids[k].type == kVoxel && (D.only < 0 || k == D.only)) drr_voxel_solid(D.solids[k], gs, ge, L);
   double t = 0.0;
-  for (int j = 0; j < D.n_w; ++j) {
+  const int n_w = D.n_w_dev ? (int)*D.n_w_dev : D.n_w;
+  for (int j = 0; j < n_w; ++j) {
     const int b = D.w_bin[j];
     double mu = 0.0;
     for (int m = 0; m < D.n_mat; ++m) {
@@ -1350,6 +1353,73 @@ __global__ __launch_bounds__(kDrrBlock) void drr_kernel(DrrParams D) {
     t += (double)D.w[j] * exp(-mu);
   }
   D.image[px] = (float)t;
+}
+
+// ---------------------------------------------------------------------------
+// the ray tracer without allocations (ggh_drr_noalloc): the spectrum weights
+// are built on the device into a file-scope staging array instead of buffers
+// allocated per call.  Layout in 32-bit words: [0] the weight count, [64 ..
+// 64 + kNoAllocMaxBins) the energy bins, then the FP32 weights.  Every call
+// on one GPU shares the array.
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocMaxBins = 1024;
+constexpr int kScratchBins = 64, kScratchWeights = kScratchBins + kNoAllocMaxBins;
+constexpr int kScratchWords = kScratchWeights + kNoAllocMaxBins;
+static __device__ __align__(256) uint32_t scratch[kScratchWords];
+
+// BinarySearch over the whole energy grid (GGEMSMathAlgorithms.hh:46-60)
+__device__ inline int grid_bin(const float* a, int size, float key) {
+  int low = 0, high = size - 1;
+  while (high - low > 1) {
+    const int mid = (high + low) / 2;
+    if (a[mid] == key) return mid;
+    if (a[mid] < key) low = mid;
+    else high = mid;
+  }
+  return low;
+}
+
+// The probability w_b of each energy bin under the source's sampling, as the
+// host builds it for ggh_drr: thread k sums bin k's share of every spectrum
+// interval in interval order (the host's order of additions, so the FP64
+// sums are identical), then one lane keeps the bins with weight, in order.
+constexpr int kWeightsBlock = 256;
+__global__ __launch_bounds__(kWeightsBlock) void drr_weights_kernel(const float* eb, int nb, const float* e, int n_e,
+                                                                   const float* f, int n_f, int is_interp,
+                                                                   uint32_t* stage) {
+  __shared__ double W[kNoAllocMaxBins];
+  for (int k = threadIdx.x; k < nb; k += kWeightsBlock) {
+    double acc = 0.0;
+    for (int idx = 0; idx + 1 < n_f && idx + 1 < n_e; ++idx) {
+      const double p = (double)f[idx + 1] - (double)f[idx];
+      if (!(p > 0.0)) continue;
+      const double a = e[idx], b = e[idx + 1];
+      if (!is_interp || !(b > a)) {
+        if (grid_bin(eb, nb, e[idx]) == k) acc += p;
+        continue;
+      }
+      if (k + 1 < nb) {
+        const double lo = k == 0 ? -INFINITY : (double)eb[k];
+        const double hi = k + 2 == nb ? INFINITY : (double)eb[k + 1];
+        const double ov = (hi < b ? hi : b) - (a < lo ? lo : a);  // std::min(b, hi) - std::max(a, lo)
+        if (ov > 0.0) acc += p * ov / (b - a);
+      }
+    }
+    W[k] = acc;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t* bins = reinterpret_cast<int32_t*>(stage + kScratchBins);
+    float* w = reinterpret_cast<float*>(stage + kScratchWeights);
+    uint32_t n = 0;
+    for (int k = 0; k < nb; ++k)
+      if (W[k] > 0.0) {
+        bins[n] = k;
+        w[n] = (float)W[k];
+        ++n;
+      }
+    stage[0] = n;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1543,6 +1613,8 @@ struct ggh_context {
   DevBuf<DevSolid> acq_solids;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
+  // this device's address of the file-scope staging array `scratch`
+  uint32_t* scratch = nullptr;
 };
 
 namespace {
@@ -1550,6 +1622,27 @@ namespace {
 int check_ctx(ggh_context* c) {
   if (!c) return fail(-1, "null context");
   HIP_TRY(hipSetDevice(c->device));
+  return 0;
+}
+
+// The device address of `scratch` on `device`, which must be the current
+// device: each GPU holds its own copy of the array.  Resolved once per
+// device; host code passes this address, never the array's name, to kernels.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch[kMaxDevices] = {};
+
+int scratch_address(int device, uint32_t** out) {
+  if (device < 0 || device >= kMaxDevices) return fail(-2, "device index beyond the staging-array table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(-100 - (int)e, std::string("staging array address: ") + hipGetErrorString(e));
+    g_scratch[device] = static_cast<uint32_t*>(p);
+  }
+  *out = g_scratch[device];
   return 0;
 }
 
@@ -1854,7 +1947,8 @@ int ggh_context_create(int device, ggh_context** out) {
     delete c;
     return fail(-100 - (int)e, std::string("device query / stream / event creation: ") + hipGetErrorString(e));
   }
-  int rc = dev_alloc(c->stats, 8);
+  int rc = scratch_address(device, &c->scratch);
+  if (!rc) rc = dev_alloc(c->stats, 8);
   if (!rc) rc = dev_alloc(c->rng, (size_t)5 * GGH_MAX_SLOTS);
   if (!rc) rc = dev_alloc(c->queue, 1);
   if (rc) {
@@ -2611,6 +2705,30 @@ int ggh_drr(ggh_context* c, int solid, int projection, float* image) {
   dw.release();
   dimg.release();
   if (e != hipSuccess) return fail(-100 - (int)e, std::string("ray tracer: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int ggh_drr_noalloc(ggh_context* c, int solid, int projection, float* image, void* stream) {
+  DrrParams D;
+  int rc = drr_setup(c, solid, projection, image, false, D);
+  if (rc) return rc;
+  if (c->nbins > kNoAllocMaxBins)
+    return fail(-2, "ggh_drr_noalloc stages the weights of at most 1024 energy bins (the physics tables have " +
+                        std::to_string(c->nbins) + "); ggh_drr has no such limit");
+  if (!c->scratch) return fail(-3, "the staging array was not resolved for this device");
+  const hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  uint32_t* stage = c->scratch;
+  hipLaunchKernelGGL(drr_weights_kernel, dim3(1), dim3(kWeightsBlock), 0, s, c->ebins.p, c->nbins, c->spectrum.p,
+                     (int)c->spectrum_host.size(), c->cdf.p, (int)c->cdf_host.size(), c->is_interp, stage);
+  const DevSolid& box = c->solids[solid];
+  const size_t npx = (size_t)box.ne[0] * box.ne[1];
+  D.w_bin = reinterpret_cast<const int32_t*>(stage + kScratchBins);
+  D.w = reinterpret_cast<const float*>(stage + kScratchWeights);
+  D.n_w_dev = stage;
+  D.image = image;
+  hipLaunchKernelGGL(drr_kernel, dim3((unsigned)((npx + kDrrBlock - 1) / kDrrBlock)), dim3(kDrrBlock), 0, s, D);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(-100 - (int)e, std::string("ray tracer (no allocation): ") + hipGetErrorString(e));
   return 0;
 }
 

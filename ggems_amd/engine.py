@@ -130,6 +130,7 @@ def library():
             "ggh_replay_acquisition": ([vp], C.c_int),
             "ggh_autotune": ([vp, C.c_uint64, C.POINTER(LaunchDesc)], C.c_int),
             "ggh_drr": ([vp, C.c_int, C.c_int, vp], C.c_int),
+            "ggh_drr_noalloc": ([vp, C.c_int, C.c_int, vp, vp], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -467,6 +468,17 @@ class Engine:
         img = np.zeros((ny, nx), np.float32)
         _check(library().ggh_drr(self.h, solid, int(projection), _ptr(img)))
         return img
+
+    def synchronize(self):
+        """waits for the engine's stream"""
+        _check(library().ggh_synchronize(self.h))
+
+    def drr_noalloc(self, solid, image_ptr, projection=-1, stream=None):
+        """ggh_drr into the device buffer at image_ptr (rows x cols float32),
+        queued on `stream` (a hipStream_t handle; None: the engine's own)
+        without allocating; returns before the image is ready"""
+        _check(library().ggh_drr_noalloc(self.h, solid, int(projection), C.c_void_p(image_ptr),
+                                         None if stream is None else C.c_void_p(stream)))
 
     def autotune(self, n_trial):
         """fastest launch shape over a trial run; call after seed() and before any run"""

@@ -231,6 +231,19 @@ int ggh_replay_acquisition(ggh_context* ctx);
  * No counterpart in GGEMS: it feeds the scatter surrogate. */
 int ggh_drr(ggh_context* ctx, int solid, int projection, float* image);
 
+/* ggh_drr without allocating: no counterpart in GGEMS.  For callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops).  `image` is a device buffer of the box solid's rows x cols floats
+ * on the context's device; the two kernels (spectrum weights, ray tracer)
+ * are queued on `stream` (a hipStream_t; null: the context's stream) and the
+ * call returns without waiting.  The weights are staged in one array of
+ * 8,448 bytes per GPU, shared by every call on that GPU: calls that overlap
+ * on one GPU race.  Limits: at most 1,024 energy bins in the physics tables,
+ * and the solid table must already be on the device (after a ggh_run,
+ * ggh_run_acquisition or ggh_drr since the last solid was added); otherwise
+ * the call fails and allocates nothing.  Same image as ggh_drr, bit for bit. */
+int ggh_drr_noalloc(ggh_context* ctx, int solid, int projection, float* image, void* stream);
+
 /* Launch autotuner: times ggh_run(n_trial) over a grid of launch shapes
  * (blocks per compute unit, round steps, interaction batch) and keeps the
  * fastest.  Call after ggh_seed and before any run: it restores the seeded
