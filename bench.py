@@ -257,6 +257,9 @@ def main():
                     help="navigation passes per round for stopped photons (0: engine default)")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--general-kernel", action="store_true", help="run the general kernel variant")
+    ap.add_argument("--sampling", choices=("reference", "tau"), default="reference",
+                    help="voxel-step sampling: GGEMS's three exponentials per step (reference) or one "
+                         "exponential number of mean free paths per flight (tau)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run a plain device copy of this size (calibrates profiler byte counters)")
     ap.add_argument("--workload", choices=("voxel_ct", "mesh_ct", "acquisition", "surrogate"), default="voxel_ct")
@@ -302,6 +305,7 @@ def main():
     eng = g.engine()
     eng.set_launch(args.round_steps, args.interact_batch, args.blocks_per_cu, args.general_kernel,
                    args.cold_passes)
+    eng.set_sampling(args.sampling)
     for _ in range(args.warmup):
         eng.run(args.photons_per_step)
     eng.reset_tallies()
@@ -342,6 +346,7 @@ def main():
                         "120 kVp Kramers + 2.5 mm Al point cone beam at SID 600 mm, flat 256x256 GOS detector "
                         "(1 mm) at SDD 1000 mm, Compton+PE+Rayleigh, FP64 dose+uncertainty",
             "photons_per_step": args.photons_per_step,
+            "sampling": args.sampling,
             "voxel_steps_per_primary": s,
             "interactions_per_primary": i,
             "detector_interactions_per_primary": d,

@@ -104,7 +104,7 @@ struct RunParams {
   const float* ebins;
   float ebin_l2min, ebin_inv;  // energy_bin's guess: log2 of the first bin, (nbins - 1) / log2 span
   const float4* tab_sig;      // [n_mat*nbins] {sigma of proc[0], proc[1], proc[2], photon cut}
-  const float4* tab_rcp;      // [n_mat*nbins] {RN(1/sigma) of proc[0..2], 0}
+  const float4* tab_rcp;      // [n_mat*nbins] {RN(1/sigma) of proc[0..2], RN(1/sigma_tot)}
   const float* cs_r;          // [n_mat*nbins] Rayleigh sigma of the material (element selection)
   const float* cs_atom_r;     // [101*nbins]
   const int32_t* n_elem;
@@ -140,7 +140,6 @@ struct RunParams {
   int32_t round_steps;        // voxel steps a wave makes at least per round
   int32_t cold_passes;        // navigation passes per round
   int32_t interact_batch;
-  uint32_t counter_bits;      // lane counters are flushed once one reaches 2^counter_bits
   uint32_t xchg_offset;       // byte offset of the lane-state exchange area in dynamic LDS
   VoxGeom vox0;               // the only voxel solid, when there is exactly one
   // CT acquisition: projection p places the source at acq_src[p] and box
@@ -195,10 +194,20 @@ struct Lane {
   uint32_t todo;
   uint32_t slot;
   uint32_t proj;  // current projection of an acquisition
+  float tau;      // tau sampling: mean free paths left in the current flight
 };
 
+// Run counters of a wave, 64-bit in LDS rather than per lane in registers
+// (the kernel's scarce resource): a counting site adds the number of its
+// active lanes through the first of them.  Each wave owns its counters, and
+// a wave's LDS operations complete in program order, so no update is lost.
+enum { kCntPrimaries = 0, kCntInteractions, kCntDetectorSteps, kCntDetectorInteractions, kCntHops, kNumCounters };
 struct Counters {
-  uint32_t primaries, interactions, detector_steps, detector_interactions, hops;
+  unsigned long long* w;  // this wave's kNumCounters counters in LDS
+  __device__ inline void add(int k) const {
+    const uint64_t m = __ballot(1);
+    if (__lane_id() == __ffsll((unsigned long long)m) - 1) w[k] += (unsigned long long)__popcll(m);
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -262,6 +271,17 @@ __device__ inline void draw_logs(const RunParams& P, Rng& r, float* lg) {
   if (np > 0) lg[0] = neg_log_uniform(r.next_sum());
   if (np > 1) lg[1] = neg_log_uniform(r.next_sum());
   if (np > 2) lg[2] = neg_log_uniform(r.next_sum());
+}
+
+// Tau sampling (opt-in; no counterpart in GGEMS, which draws three competing
+// exponentials per voxel step, GGEMSPhotonNavigator.hh:57-93): one number of
+// mean free paths -log(u) per flight, drawn when the photon enters a voxel
+// solid and after each interaction that leaves it alive.  u = 0 gives +inf
+// (the photon never interacts), as -logf(0) does.
+__device__ inline float draw_tau(Rng& r) {
+  const uint32_t s = r.next_sum();
+  const float t = neg_log_uniform(s);
+  return s == 0u ? __builtin_inff() : t;
 }
 
 // KleinNishinaComptonSampleSecondaries (GGEMSComptonScatteringModels.hh:47-132)
@@ -481,6 +501,48 @@ __device__ inline bool hot_step(const RunParams& P, const Tabs& T, const VoxGeom
   return true;
 }
 
+// The voxel step of tau sampling: the walk, tolerances and outcomes of
+// hot_step, with the interaction decided by the flight's tau instead of
+// three fresh exponentials.  A step whose boundary distance db costs
+// a = sigma_tot * db <= tau crosses the voxel and spends a; otherwise the
+// photon interacts at tau / sigma_tot (the IEEE quotient) and the process
+// is chosen when the interaction is resolved.  sigma_tot = (sigma_0 +
+// sigma_1) + sigma_2 in activation order, its reciprocal in the w of the
+// second table entry.
+template <bool kTrack>
+__device__ inline bool hot_step_tau(const RunParams& P, const Tabs& T, const VoxGeom& G, Lane& L) {
+  int vx, vy, vz;
+  if (!voxel_of(L.p, G, vx, vy, vz)) {
+    L.mode = kExit;
+    return false;
+  }
+  const uint32_t label = G.labels[vx + vy * G.nx + vz * G.nxy];
+  const Box vox = voxel_box(G, vx, vy, vz);
+  safety_inside(L.p, vox);
+  const float db = distance_to_aabb(L.p, L.ray, vox);
+  const int gmat = G.mat_offset + (int)label;
+  const int ti = gmat * P.nbins + L.eid;
+  const float4 sig = T.sig[ti];
+  const float rtot = T.rcp[ti].w;
+  const float tot = (sig.x + sig.y) + sig.z;
+  const float a = tot * db;
+  const bool cross = !(L.tau < a);
+  const float next = cross ? db + kTol : div_rn(L.tau, tot, rtot);
+  L.tau = cross ? L.tau - a : L.tau;
+  if (kTrack && cross && P.ptrack && G.solid == P.dose_solid) {
+    int id = dosel_index(P, L.p);
+    if (id >= 0) atomicAdd(&P.ptrack[id], 1);
+  }
+  L.p = add(L.p, scl(L.ray.d, next));
+  safety_outside(L.p, vox);
+  const bool out = !in_bounds(L.p, G);
+  const bool inter = !out & !cross;
+  const bool cut = !out & !inter & (L.e <= sig.w);  // photon_energy_cut_[material]
+  L.mode = out ? kExit : (inter ? kInteract : (cut ? kCut : kInVoxel));
+  L.pend = inter ? (gmat << 2) : L.pend;
+  return true;
+}
+
 // The same walk with no interaction (sigma == 0): records the voxel indices.
 __device__ inline bool walk_step(const VoxGeom& G, f3& p, const Ray& ray, int32_t* out, int& n, int max_out) {
   int vx, vy, vz;
@@ -503,7 +565,7 @@ __device__ inline bool walk_step(const VoxGeom& G, f3& p, const Ray& ray, int32_
 // kAcq: the module sits where the lane's projection puts it and scores into
 // that projection's image.
 template <bool kAcq>
-__device__ void track_box(const RunParams& P, const Tabs& T, const DevSolid& s, Lane& L, Counters& c) {
+__device__ void track_box(const RunParams& P, const Tabs& T, const DevSolid& s, Lane& L, const Counters& c) {
   const Mat34& m = kAcq ? P.acq_box[L.proj * P.n_box + s.box_index] : s.m;
   f3 lp = g2l_pos(m, L.p);
   Ray ld;
@@ -517,7 +579,7 @@ __device__ void track_box(const RunParams& P, const Tabs& T, const DevSolid& s, 
     float lg[3];
     draw_logs<-1>(P, L.r, lg);
     float next = next_interaction<-1>(P, T.sig[ti], T.rcp[ti], lg, proc);
-    ++c.detector_steps;
+    c.add(kCntDetectorSteps);
     safety_inside(lp, b);
     float db = distance_to_aabb(lp, ld, b);
     if (db <= next) {
@@ -529,7 +591,7 @@ __device__ void track_box(const RunParams& P, const Tabs& T, const DevSolid& s, 
     if (!in_aabb(lp, b)) break;  // leaves the module: back to navigation
     if (L.e < s.threshold) alive = false;
     if (proc != kTransportation) {
-      ++c.detector_interactions;
+      c.add(kCntDetectorInteractions);
       float e0 = L.e;
       f3 dir = ld.d;
       if (!discrete(P, T.ebins, proc, gmat, L.e, L.eid, dir, L.r)) alive = false;
@@ -792,32 +854,14 @@ __device__ int find_and_project(const RunParams& P, const Tabs& T, f3& pos, f3 d
   return solid;
 }
 
-__device__ inline unsigned long long wave_sum(unsigned long long v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-// The lane counters are 32-bit (registers are the kernel's scarce resource).
-// When a lane starts a primary and one of its counters has reached
-// 2^counter_bits (2^31 unless a test lowers it), the lane adds all of them to
-// the 64-bit run counters and clears them.  One history adds far less than
-// 2^31 to any counter, so none can wrap.  At the end each wave adds its
-// lanes' remainders (sums of 64 values below 2^32, in 64 bits).
-__device__ inline void flush_lane_counters(const RunParams& P, Counters& c) {
-  const uint32_t v[5] = {c.primaries, c.interactions, c.detector_steps, c.detector_interactions, c.hops};
-  const int slots[5] = {0, 2, 3, 4, 5};
-  for (int k = 0; k < 5; ++k)
-    if (v[k]) atomicAdd(&P.stats[slots[k]], (unsigned long long)v[k]);
-  c = Counters{0, 0, 0, 0, 0};
-}
-
+// a wave's counters into the run statistics (ggh_stats order: primaries,
+// voxel steps, interactions, detector steps, detector interactions, hops)
 __device__ inline void flush_wave_counters(const RunParams& P, const Counters& c) {
-  const unsigned long long sums[5] = {wave_sum(c.primaries), wave_sum(c.interactions),
-                                      wave_sum(c.detector_steps), wave_sum(c.detector_interactions),
-                                      wave_sum(c.hops)};
-  const int slots[5] = {0, 2, 3, 4, 5};
-  for (int k = 0; k < 5; ++k)
-    if (__lane_id() == 0 && sums[k]) atomicAdd(&P.stats[slots[k]], sums[k]);
+  const int k = __lane_id();
+  if (k < kNumCounters) {
+    const unsigned long long v = c.w[k];
+    if (v) atomicAdd(&P.stats[k == kCntPrimaries ? 0 : k + 1], v);
+  }
 }
 
 __device__ inline void load_stream(const RunParams& P, uint32_t slot, Rng& r) {
@@ -838,9 +882,24 @@ __device__ inline void store_stream(const RunParams& P, uint32_t slot, const Rng
 
 // The discrete process a voxel step stopped on, then the energy cut
 // (TrackThroughGGEMSVoxelizedSolid.cl:238-310).
-__device__ inline void resolve_interaction(const RunParams& P, const Tabs& T, Lane& L, Counters& c) {
-  const int proc = L.pend & 3, gmat = L.pend >> 2;
-  ++c.interactions;
+template <int NP, bool kTau>
+__device__ inline void resolve_interaction(const RunParams& P, const Tabs& T, Lane& L, const Counters& c) {
+  const int gmat = L.pend >> 2;
+  int proc = L.pend & 3;
+  if (kTau) {
+    // the process, in proportion to its left-bin cross section: the first
+    // whose running sum in activation order exceeds u * sigma_tot
+    const int np = NP >= 0 ? NP : P.n_proc;
+    const float4 s = T.sig[gmat * P.nbins + L.eid];
+    const float x = L.r.uniform() * ((s.x + s.y) + s.z);
+    proc = P.proc[np > 0 ? np - 1 : 0];
+    if (np > 1) {
+      const float c0 = s.x;
+      if (x < c0) proc = P.proc[0];
+      else if (np > 2 && x < c0 + s.y) proc = P.proc[1];
+    }
+  }
+  c.add(kCntInteractions);
   const float e0 = L.e;
   f3 dir = L.ray.d;
   bool alive = discrete(P, T.ebins, proc, gmat, L.e, L.eid, dir, L.r);
@@ -856,6 +915,7 @@ __device__ inline void resolve_interaction(const RunParams& P, const Tabs& T, La
     alive = false;
   }
   L.mode = alive ? kInVoxel : kNeedPrimary;
+  if (kTau && alive) L.tau = draw_tau(L.r);
 }
 
 // primaries of particle slot s in a run laid out as {n_batches, base, rem}:
@@ -876,9 +936,9 @@ __device__ inline void next_projection(const RunParams& P, Lane& L) {
 
 // the cold work of one lane, one state at a time in an order that lets a
 // lane pass through several states in one call
-template <bool kSingle, bool kAcq>
-__device__ inline void cold_step(const RunParams& P, const Tabs& T, Lane& L, Counters& c) {
-  if (L.mode == kInteract) resolve_interaction(P, T, L, c);
+template <int NP, bool kSingle, bool kAcq, bool kTau>
+__device__ inline void cold_step(const RunParams& P, const Tabs& T, Lane& L, const Counters& c) {
+  if (L.mode == kInteract) resolve_interaction<NP, kTau>(P, T, L, c);
   if (L.mode == kCut) {
     if (L.solid == P.dose_solid) dose_record(P, L.e, L.p);
     L.mode = kNeedPrimary;
@@ -926,11 +986,8 @@ __device__ inline void cold_step(const RunParams& P, const Tabs& T, Lane& L, Cou
       L.mode = kNeedSlot;  // empty slot: take another one on the next pass
     } else {
       --L.todo;
-      if (((c.primaries | c.interactions | c.detector_steps | c.detector_interactions | c.hops) >>
-           P.counter_bits) != 0u)
-        flush_lane_counters(P, c);
       sample_primary<kAcq>(P, T, kAcq ? P.acq_src[L.proj] : P.src_m, L.r, L.p, L.ray.d, L.e, L.eid);
-      ++c.primaries;
+      c.add(kCntPrimaries);
       L.scatter = 0;
       L.hop = 0;
       L.mode = kHop;
@@ -942,7 +999,7 @@ __device__ inline void cold_step(const RunParams& P, const Tabs& T, Lane& L, Cou
       return;
     }
     ++L.hop;
-    ++c.hops;
+    c.add(kCntHops);
     const int k = find_and_project<kAcq>(P, T, L.p, L.ray.d, L.proj);
     if (k < 0) {
       L.mode = kNeedPrimary;
@@ -954,6 +1011,7 @@ __device__ inline void cold_step(const RunParams& P, const Tabs& T, Lane& L, Cou
       L.p = g2l_pos(s.m, L.p);
       L.ray.set(g2l_dir(s.m, L.ray.d));
       L.mode = kInVoxel;
+      if (kTau) L.tau = draw_tau(L.r);
     } else if (!kSingle && s.type == kMesh) {
       // ggh_run launches the single-voxel-solid variant only on scenes
       // without meshes, which keeps the mesh loop (and even this test) out
@@ -974,6 +1032,7 @@ __device__ inline void cold_step(const RunParams& P, const Tabs& T, Lane& L, Cou
 constexpr int kWaves = kBlock / 64;
 constexpr size_t kXchgBytes = (size_t)kBlock * 12 * sizeof(uint32_t);
 
+template <bool kTau>
 __device__ inline void exchange_lane(Lane& L, uint4* s_x, uint32_t pos) {
   const uint4* src = s_x + (size_t)threadIdx.x * 3;
   uint4* dst = s_x + (size_t)pos * 3;
@@ -990,8 +1049,15 @@ __device__ inline void exchange_lane(Lane& L, uint4* s_x, uint32_t pos) {
   __syncthreads();
   dst[0] = make_uint4(__float_as_uint(L.ray.inv.y), __float_as_uint(L.ray.inv.z), __float_as_uint(L.e),
                       (uint32_t)L.eid);
-  dst[1] = make_uint4((uint32_t)L.mode, (uint32_t)L.solid, (uint32_t)L.hop, (uint32_t)L.scatter);
-  dst[2] = make_uint4((uint32_t)L.pend, L.todo, L.slot, L.proj);
+  if (kTau) {
+    // mode (3 bits), scatter (1 bit) and hop (<= GGH_MAX_HOPS) share a word to make room for tau
+    dst[1] = make_uint4((uint32_t)L.mode | ((uint32_t)L.scatter << 3) | ((uint32_t)L.hop << 4), (uint32_t)L.solid,
+                        (uint32_t)L.pend, L.todo);
+    dst[2] = make_uint4(L.slot, L.proj, __float_as_uint(L.tau), 0u);
+  } else {
+    dst[1] = make_uint4((uint32_t)L.mode, (uint32_t)L.solid, (uint32_t)L.hop, (uint32_t)L.scatter);
+    dst[2] = make_uint4((uint32_t)L.pend, L.todo, L.slot, L.proj);
+  }
   __syncthreads();
   a = src[0];
   b = src[1];
@@ -1000,14 +1066,26 @@ __device__ inline void exchange_lane(Lane& L, uint4* s_x, uint32_t pos) {
   L.ray.inv.z = __uint_as_float(a.y);
   L.e = __uint_as_float(a.z);
   L.eid = (int32_t)a.w;
-  L.mode = (int32_t)b.x;
-  L.solid = (int32_t)b.y;
-  L.hop = (int32_t)b.z;
-  L.scatter = (int32_t)b.w;
-  L.pend = (int32_t)d.x;
-  L.todo = d.y;
-  L.slot = d.z;
-  L.proj = d.w;
+  if (kTau) {
+    L.mode = (int32_t)(b.x & 7u);
+    L.scatter = (int32_t)((b.x >> 3) & 1u);
+    L.hop = (int32_t)(b.x >> 4);
+    L.solid = (int32_t)b.y;
+    L.pend = (int32_t)b.z;
+    L.todo = b.w;
+    L.slot = d.x;
+    L.proj = d.y;
+    L.tau = __uint_as_float(d.z);
+  } else {
+    L.mode = (int32_t)b.x;
+    L.solid = (int32_t)b.y;
+    L.hop = (int32_t)b.z;
+    L.scatter = (int32_t)b.w;
+    L.pend = (int32_t)d.x;
+    L.todo = d.y;
+    L.slot = d.z;
+    L.proj = d.w;
+  }
   __syncthreads();
 }
 
@@ -1022,10 +1100,13 @@ __device__ inline uint32_t lds_load_relaxed(uint32_t* p) {
 // and module placement); kRegroup: the scene has a voxel solid, and blocks
 // work in rounds with regrouping (below).  Without a voxel solid every
 // photon is navigated and tracked by its lane's state machine alone.
-template <int NP, bool kSingle, bool kLDS, bool kTrack, bool kAcq, bool kRegroup>
+template <int NP, bool kSingle, bool kLDS, bool kTrack, bool kAcq, bool kRegroup, bool kTau = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWavesPerEU)))
 void transport_kernel(RunParams P) {
   extern __shared__ float4 s_tab[];
+  __shared__ unsigned long long s_counts[kBlock / 64][kNumCounters];
+  if (threadIdx.x < (kBlock / 64) * kNumCounters) (&s_counts[0][0])[threadIdx.x] = 0ull;
+  __syncthreads();
   Tabs T = {P.tab_sig, P.tab_rcp, P.ebins, P.solids, P.spectrum, P.cdf};
   if (kLDS) {
     // LDS layout (the host sizes it with the same arithmetic, lds_layout):
@@ -1074,7 +1155,8 @@ void transport_kernel(RunParams P) {
   L.todo = 0;
   L.slot = 0xffffffffu;
   L.proj = 0;
-  Counters c = {0, 0, 0, 0, 0};
+  L.tau = 0.0f;
+  const Counters c = {s_counts[threadIdx.x >> 6]};
   unsigned long long steps = 0;  // wave-uniform
 
   if (!kRegroup) {
@@ -1083,7 +1165,7 @@ void transport_kernel(RunParams P) {
     for (;;) {
       const bool cold = L.mode != kInVoxel && L.mode != kDone;
       if (!__any(cold)) break;
-      if (cold) cold_step<kSingle, kAcq>(P, T, L, c);
+      if (cold) cold_step<NP, kSingle, kAcq, kTau>(P, T, L, c);
     }
   } else {
     // The block works in rounds.  In each round every wave
@@ -1116,7 +1198,7 @@ void transport_kernel(RunParams P) {
       for (int pass = 0; pass < P.cold_passes; ++pass) {
         const bool cold = L.mode != kInVoxel && L.mode != kDone;
         if (!__any(cold)) break;
-        if (cold) cold_step<kSingle, kAcq>(P, T, L, c);
+        if (cold) cold_step<NP, kSingle, kAcq, kTau>(P, T, L, c);
       }
       if (lane == 0) atomicAdd(&s_cold_done[rb], 1u);
       bool ready = false;
@@ -1130,12 +1212,15 @@ void transport_kernel(RunParams P) {
         bool stepped = false;
         // the phantom's geometry: kernel arguments for the only voxel
         // solid, else the lane's solid in the (LDS-staged) solid table
-        if (L.mode == kInVoxel) stepped = hot_step<NP, kTrack>(P, T, kSingle ? P.vox0 : T.solids[L.solid].vox, L);
+        if (L.mode == kInVoxel) {
+          const VoxGeom& G = kSingle ? P.vox0 : T.solids[L.solid].vox;
+          stepped = kTau ? hot_step_tau<kTrack>(P, T, G, L) : hot_step<NP, kTrack>(P, T, G, L);
+        }
         steps += __popcll(__ballot(stepped));
         // pending interactions resolved in batches, so that scattered photons
         // keep stepping without waiting for the next round
         if (__popcll(__ballot(L.mode == kInteract)) >= P.interact_batch) {
-          if (L.mode == kInteract) resolve_interaction(P, T, L, c);
+          if (L.mode == kInteract) resolve_interaction<NP, kTau>(P, T, L, c);
         }
       }
       // regrouping: rank 0 inside a phantom, 1 stopped, 2 no work left
@@ -1165,7 +1250,7 @@ void transport_kernel(RunParams P) {
       const uint64_t mine = key == 0 ? b0 : (key == 1 ? b1 : ~(b0 | b1));
       const uint32_t pos = (key == 0 ? before0 : (key == 1 ? n_hot + before1 : n_hot + n_cold + before2)) +
                            (uint32_t)__popcll(mine & below);
-      exchange_lane(L, s_x, pos);
+      exchange_lane<kTau>(L, s_x, pos);
     }
   }
 
@@ -1597,8 +1682,8 @@ struct ggh_context {
   int cold_passes = 2;
   int interact_batch = 16;
   int blocks_per_cu = 0;  // 0: as many as fit
-  int counter_bits = 31;
   bool force_general = false;
+  int tau_mode = 0;  // voxel steps sample interactions by tau (one exponential per flight)
   unsigned last_grid = 0;
   int last_variant = -1;
   // CT acquisition
@@ -1741,7 +1826,6 @@ RunParams make_params(ggh_context* c) {
   P.round_steps = c->round_steps;
   P.cold_passes = c->cold_passes;
   P.interact_batch = c->interact_batch;
-  P.counter_bits = (uint32_t)c->counter_bits;
   P.xchg_offset = 0;
   for (const DevSolid& s : c->solids)
     if (s.type == kVoxel) {
@@ -1856,12 +1940,26 @@ int pick_launch(ggh_context* c, uint64_t slots, bool acq, Launch& L) {
   const bool special = three && single;
   L.xchg_offset = use_lds ? (uint32_t)lds : 0u;
   L.shmem = L.xchg_offset + xchg;
-  if (special)
+  // tau sampling changes only the voxel step: scenes without a voxel solid
+  // run the plain variants
+  const bool tau = c->tau_mode != 0 && regroup;
+  if (special && tau)
+    L.fn = acq ? transport_kernel<3, true, true, false, true, true, true>
+               : transport_kernel<3, true, true, false, false, true, true>;
+  else if (special)
     L.fn = acq ? transport_kernel<3, true, true, false, true, true>
                : transport_kernel<3, true, true, false, false, true>;
+  else if (three && regroup && tau)
+    L.fn = acq ? transport_kernel<3, false, true, false, true, true, true>
+               : transport_kernel<3, false, true, false, false, true, true>;
   else if (three && regroup)
     L.fn = acq ? transport_kernel<3, false, true, false, true, true>
                : transport_kernel<3, false, true, false, false, true>;
+  else if (tau)
+    L.fn = acq ? (use_lds ? transport_kernel<-1, false, true, true, true, true, true>
+                          : transport_kernel<-1, false, false, true, true, true, true>)
+               : (use_lds ? transport_kernel<-1, false, true, true, false, true, true>
+                          : transport_kernel<-1, false, false, true, false, true, true>);
   else if (acq)
     L.fn = use_lds ? (regroup ? transport_kernel<-1, false, true, true, true, true>
                               : transport_kernel<-1, false, true, true, true, false>)
@@ -2013,7 +2111,7 @@ int ggh_set_physics(ggh_context* c, const ggh_physics_desc* d) {
       }
       const size_t at = (size_t)m * nb + i;
       sig[at] = make_float4(s[0], s[1], s[2], d->photon_cut[m]);
-      rcp[at] = make_float4(y[0], y[1], y[2], 0.f);
+      rcp[at] = make_float4(y[0], y[1], y[2], host_rcp((s[0] + s[1]) + s[2]));
       cs_r[at] = d->cross_sections[(size_t)2 * nm * nb + (size_t)m * nb + i];
     }
   c->nbins = nb;
@@ -2429,19 +2527,26 @@ int ggh_reset_tallies(ggh_context* c) {
   return 0;
 }
 
+int ggh_set_sampling(ggh_context* c, int32_t mode) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (mode != GGH_SAMPLING_REFERENCE && mode != GGH_SAMPLING_TAU) return fail(-2, "unknown sampling mode");
+  drop_graph(c);  // a captured acquisition holds the kernel of the previous mode
+  c->tau_mode = mode == GGH_SAMPLING_TAU;
+  return 0;
+}
+
 int ggh_set_launch(ggh_context* c, const ggh_launch_desc* d) {
   int rc = check_ctx(c);
   if (rc) return rc;
   if (!d || d->round_steps < 0 || d->round_steps > 1024 || d->cold_passes < 0 || d->cold_passes > 16 ||
-      d->interact_batch < 0 || d->interact_batch > 64 || d->blocks_per_cu < 0 || d->blocks_per_cu > 64 ||
-      d->counter_bits < 0 || d->counter_bits > 31)
+      d->interact_batch < 0 || d->interact_batch > 64 || d->blocks_per_cu < 0 || d->blocks_per_cu > 64)
     return fail(-2, "invalid launch parameters");
   c->round_steps = d->round_steps ? d->round_steps : 20;
   c->cold_passes = d->cold_passes ? d->cold_passes : 2;
   c->interact_batch = d->interact_batch ? d->interact_batch : 16;
   c->blocks_per_cu = d->blocks_per_cu;
   c->force_general = d->general_kernel != 0;
-  c->counter_bits = d->counter_bits ? d->counter_bits : 31;
   return 0;
 }
 

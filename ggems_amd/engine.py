@@ -66,7 +66,7 @@ class XRaySourceDesc(C.Structure):
 
 class LaunchDesc(C.Structure):
     _fields_ = [("round_steps", C.c_int32), ("cold_passes", C.c_int32), ("interact_batch", C.c_int32),
-                ("blocks_per_cu", C.c_int32), ("general_kernel", C.c_int32), ("counter_bits", C.c_int32)]
+                ("blocks_per_cu", C.c_int32), ("general_kernel", C.c_int32)]
 
 
 class AcquisitionDesc(C.Structure):
@@ -131,6 +131,7 @@ def library():
             "ggh_autotune": ([vp, C.c_uint64, C.POINTER(LaunchDesc)], C.c_int),
             "ggh_drr": ([vp, C.c_int, C.c_int, vp], C.c_int),
             "ggh_drr_noalloc": ([vp, C.c_int, C.c_int, vp, vp], C.c_int),
+            "ggh_set_sampling": ([vp, C.c_int32], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -399,12 +400,10 @@ class Engine:
         _check(library().ggh_debug_primaries(self.h, n, _ptr(pos), _ptr(dir), _ptr(e)))
         return pos, dir, e
 
-    def set_launch(self, round_steps=0, interact_batch=0, blocks_per_cu=0, general_kernel=False, cold_passes=0,
-                   counter_bits=0):
-        """launch shape of run(); zero keeps a default (round_steps 20, cold_passes 2, interact_batch 16,
-        counter_bits 31)"""
+    def set_launch(self, round_steps=0, interact_batch=0, blocks_per_cu=0, general_kernel=False, cold_passes=0):
+        """launch shape of run(); zero keeps a default (round_steps 20, cold_passes 2, interact_batch 16)"""
         d = LaunchDesc(int(round_steps), int(cold_passes), int(interact_batch), int(blocks_per_cu),
-                       int(bool(general_kernel)), int(counter_bits))
+                       int(bool(general_kernel)))
         _check(library().ggh_set_launch(self.h, C.byref(d)))
 
     def last_grid(self):
@@ -468,6 +467,12 @@ class Engine:
         img = np.zeros((ny, nx), np.float32)
         _check(library().ggh_drr(self.h, solid, int(projection), _ptr(img)))
         return img
+
+    def set_sampling(self, mode):
+        """"reference" (GGEMS's three exponentials per voxel step, the
+        default) or "tau" (one exponential number of mean free paths per
+        flight; same distributions, different random streams)"""
+        _check(library().ggh_set_sampling(self.h, {"reference": 0, "tau": 1}[mode]))
 
     def synchronize(self):
         """waits for the engine's stream"""

@@ -174,10 +174,24 @@ typedef struct {
                               them to the navigation passes) */
   int32_t blocks_per_cu;   /* cap on persistent blocks per compute unit (default: as many as fit) */
   int32_t general_kernel;  /* nonzero: the general kernel variant even where the specialised one applies */
-  int32_t counter_bits;    /* the 32-bit per-lane run counters are added to the 64-bit run statistics once
-                              one reaches 2^counter_bits (1..31, default 31; lower values test the flush) */
 } ggh_launch_desc;
 int ggh_set_launch(ggh_context* ctx, const ggh_launch_desc* desc);
+
+/* How a voxel step samples its interaction (no counterpart in GGEMS).
+ * GGH_SAMPLING_REFERENCE (the default) draws the three competing
+ * exponentials of GetPhotonNextInteraction in every voxel step
+ * (include/GGEMS/navigators/GGEMSPhotonNavigator.hh:57-93), so every history
+ * is GGEMS's.  GGH_SAMPLING_TAU draws one exponential number of mean free
+ * paths per flight (on entering a voxel solid and after each interaction)
+ * and spends sigma_tot * db of it per crossed voxel; the process is chosen
+ * by one uniform in proportion to its left-bin cross section.  The voxel
+ * walk, tolerances, interactions and tallies are unchanged and the
+ * distributions per voxel are the same; the random streams, and so the
+ * individual histories, differ from GGEMS's.  Boxes and meshes keep the
+ * reference sampling. */
+#define GGH_SAMPLING_REFERENCE 0
+#define GGH_SAMPLING_TAU 1
+int ggh_set_sampling(ggh_context* ctx, int32_t mode);
 /* grid and kernel variant of the last ggh_run: 0 specialised (three processes,
  * one voxel solid), 1 general, 2 three processes over several solids */
 int ggh_last_grid(ggh_context* ctx, int32_t* blocks, int32_t* block_size, int32_t* variant);

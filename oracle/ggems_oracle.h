@@ -136,6 +136,11 @@ typedef struct {
   const uint8_t* z;
   const float* atom_density;
   const float* photon_cut;
+  /* 0: the reference's sampling, three competing exponentials per voxel
+   * step; 1: one exponential number of mean free paths per flight, spent
+   * voxel by voxel (opt-in, no counterpart in the reference; see
+   * track_voxel) */
+  int tau_mode;
 } ggo_physics;
 
 typedef struct {

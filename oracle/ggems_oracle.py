@@ -93,6 +93,7 @@ class Physics(C.Structure):
         ("z", C.c_void_p),
         ("atom_density", C.c_void_p),
         ("photon_cut", C.c_void_p),
+        ("tau_mode", C.c_int),
     ]
 
 

@@ -477,6 +477,35 @@ static inline void dose_record(const ggo_solid* s, float edep, f3 p) {
 static inline f3 add3(f3 a, f3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
 static inline f3 scale3(f3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
 
+/* Opt-in sampling with one number of mean free paths per flight (tau_mode;
+ * no counterpart in the reference, which draws three competing exponentials
+ * in every voxel step, GGEMSPhotonNavigator.hh:57-93).  tau = -log(u) is
+ * drawn when the photon enters a voxel solid and after each interaction that
+ * leaves it alive.  A voxel step whose boundary distance db costs
+ * a = sigma_tot * db <= tau crosses the voxel and spends a; otherwise the
+ * photon interacts at tau / sigma_tot, the process chosen by one uniform in
+ * proportion to its left-bin cross section.  Per voxel this gives the
+ * reference's probabilities: crossing exp(-sigma_tot db), process i with
+ * sigma_i / sigma_tot, and an exponential depth truncated at db.  The
+ * voxel walk, the tolerances and the interactions are unchanged. */
+static inline float total_cs(const ggo_physics* ph, int gmat, int eid) {
+  float t = 0.0f;
+  for (int i = 0; i < ph->n_processes; ++i) t += ph->cs[ph->process_id[i]][eid + ph->nbins * gmat];
+  return t;
+}
+
+static inline int select_process(const ggo_physics* ph, int gmat, int eid, float tot, uint32_t* rng) {
+  const float x = ggo_kiss_uniform(rng) * tot;
+  float c = 0.0f;
+  for (int i = 0; i + 1 < ph->n_processes; ++i) {
+    c += ph->cs[ph->process_id[i]][eid + ph->nbins * gmat];
+    if (x < c) return ph->process_id[i];
+  }
+  return ph->process_id[ph->n_processes - 1];
+}
+
+#define INTERACTION_PENDING (-1) /* tau mode: the process is chosen once the step stays inside */
+
 /* track_through_ggems_voxelized_solid (TrackThroughGGEMSVoxelizedSolid.cl:76-323) */
 static void track_voxel(const ggo_physics* ph, const ggo_solid* s, photon* g, uint32_t* rng,
                         ggo_stats* st, int32_t* walk, int walk_max, int* walk_n) {
@@ -486,6 +515,8 @@ static void track_voxel(const ggo_physics* ph, const ggo_solid* s, photon* g, ui
   const float* bmin = s->obb.bmin;
   const float* bmax = s->obb.bmax;
   const float* vs = s->voxel_size;
+  const int tau_mode = ph->tau_mode && !walk;
+  float tau = tau_mode ? -logf(ggo_kiss_uniform(rng)) : 0.0f;
   do {
     int vx = (int)((lp.x - bmin[0]) / vs[0]);
     int vy = (int)((lp.y - bmin[1]) / vs[1]);
@@ -504,14 +535,34 @@ static void track_voxel(const ggo_physics* ph, const ggo_solid* s, photon* g, ui
     }
     int gmat = s->mat_offset + s->labels[vid];
     if (st) st->voxel_steps++;
-    int proc;
-    float next = walk ? OUT_OF_WORLD : next_interaction(ph, g, gmat, rng, &proc);
-    if (walk) proc = TRANSPORTATION;
+    int proc = TRANSPORTATION;
+    float next = OUT_OF_WORLD, tot = 0.0f;
+    int eid = 0;
+    if (tau_mode) {
+      eid = bsearch_f(g->e, ph->energy_bins, ph->nbins);
+      g->e_index = eid;
+      tot = total_cs(ph, gmat, eid);
+    } else if (!walk) {
+      next = next_interaction(ph, g, gmat, rng, &proc);
+    }
     float vlo[3] = {bmin[0] + (float)vx * vs[0], bmin[1] + (float)vy * vs[1], bmin[2] + (float)vz * vs[2]};
     float vhi[3] = {vlo[0] + vs[0], vlo[1] + vs[1], vlo[2] + vs[2]};
     safety_inside(&lp, vlo, vhi);
     float db = dist_aabb(lp, ld, vlo, vhi);
-    if (db <= next) {
+    int cross;
+    if (tau_mode) {
+      const float a = tot * db;
+      cross = !(tau < a);
+      if (cross) {
+        tau = tau - a;
+      } else {
+        next = tau / tot;
+        proc = INTERACTION_PENDING;
+      }
+    } else {
+      cross = db <= next;
+    }
+    if (cross) {
       next = db + TOL;
       proc = TRANSPORTATION;
       if (s->dosimetry && s->photon_tracking) {
@@ -529,6 +580,7 @@ static void track_voxel(const ggo_physics* ph, const ggo_solid* s, photon* g, ui
     float e0 = g->e;
     if (proc != TRANSPORTATION) {
       if (st) st->interactions++;
+      if (proc == INTERACTION_PENDING) proc = select_process(ph, gmat, eid, tot, rng);
       g->d = ld;
       discrete(ph, g, proc, gmat, rng);
       if (proc == COMPTON || proc == RAYLEIGH) g->scatter = 1;
@@ -539,6 +591,7 @@ static void track_voxel(const ggo_physics* ph, const ggo_solid* s, photon* g, ui
       if (s->dosimetry) dose_record(s, g->e, lp);
       g->status = DEAD;
     }
+    if (tau_mode && proc != TRANSPORTATION && g->status == ALIVE) tau = -logf(ggo_kiss_uniform(rng));
   } while (g->status == ALIVE);
   g->p = l2g_pos(m, lp);
   g->d = l2g_dir(m, ld);

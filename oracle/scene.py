@@ -86,6 +86,7 @@ class Scene:
         self.cut_distance = cut_distance
         self.navigators = []   # (kind, materials, payload)
         self.source = None
+        self.tau = False       # tau sampling mode (one exponential per flight) in oracle and engine
 
     # ---- geometry -----------------------------------------------------------
     def add_phantom(self, labels, materials, voxel_size, position=(0, 0, 0), rotation=(0, 0, 0), dosimetry=None):
@@ -221,6 +222,7 @@ class Scene:
         a = self.arrays
         ph.n_elements, ph.elem_index = O.ptr(a["n_elements"]), O.ptr(a["elem_index"])
         ph.z, ph.atom_density, ph.photon_cut = O.ptr(a["z"]), O.ptr(a["atom_density"]), O.ptr(a["photon_cut"])
+        ph.tau_mode = 1 if self.tau else 0
         solids = []
         for (kind, materials, pay), off in zip(self.navigators, self.offsets):
             if kind == "voxel":
@@ -411,6 +413,8 @@ class Scene:
                     self.engine_solids.setdefault("box", []).append(k)
         S = self.source
         eng.set_xray_source(S["m"], S["aperture"], S["focal"], S["energy"], S["cdf"], S["is_interp"])
+        if self.tau:
+            eng.set_sampling("tau")
         return eng
 
     def run_engine(self, n, seed, eng=None):

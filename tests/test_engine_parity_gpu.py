@@ -199,12 +199,10 @@ def test_launch_shape_does_not_change_histories(cfg1, general):
     eng = cfg1.make_engine()
     eng.set_launch(general_kernel=general)
     base = cfg1.run_engine(n, 2024, eng)
-    # counter_bits 3: a lane adds its 32-bit counters to the 64-bit run
-    # statistics whenever one has reached 8, i.e. at almost every primary
-    for rs, cp, ib, cb in [(1, 1, 1, 0), (1024, 16, 64, 0), (7, 3, 5, 3)]:
-        eng.set_launch(round_steps=rs, cold_passes=cp, interact_batch=ib, general_kernel=general, counter_bits=cb)
+    for rs, cp, ib in [(1, 1, 1), (1024, 16, 64), (7, 3, 5)]:
+        eng.set_launch(round_steps=rs, cold_passes=cp, interact_batch=ib, general_kernel=general)
         got = cfg1.run_engine(n, 2024, eng)
-        assert got["stats"] == base["stats"], (rs, cp, ib, cb)
+        assert got["stats"] == base["stats"], (rs, cp, ib)
         assert np.array_equal(got["hist"][0], base["hist"][0]) and np.array_equal(got["hit"], base["hit"])
         # FP64 sums whose order follows the atomics
         np.testing.assert_allclose(got["edep"], base["edep"], rtol=1e-12, atol=1e-18)
@@ -215,7 +213,7 @@ def test_launch_rejects_invalid_parameters(cfg1):
     from ggems_amd.engine import EngineError
     eng = cfg1.make_engine()
     for kw in (dict(round_steps=-1), dict(round_steps=1025), dict(cold_passes=17), dict(interact_batch=65),
-               dict(blocks_per_cu=-2), dict(counter_bits=32), dict(counter_bits=-1)):
+               dict(blocks_per_cu=-2), dict(blocks_per_cu=65)):
         with pytest.raises(EngineError):
             eng.set_launch(**kw)
     eng.close()

@@ -80,34 +80,6 @@ def test_full_step_launch_invariance_and_oracle_sample(body):
     assert abs(e_got - e_ref) <= 0.02 * e_ref, (e_got, e_ref)
 
 
-def _block(a, b):
-    nz, ny, nx = a.shape
-    return a.reshape(nz // b, b, ny // b, b, nx // b, b).sum(axis=(1, 3, 5), dtype=np.float64)
-
-
-def _dose_z(x, y, b=8, min_hits=50):
-    """Two-sample z-scores of the energy deposited in blocks of b^3 dosels:
-    (E_x - E_y) / sqrt(V_x + V_y), V = the block's sum of squared deposits
-    (the variance of a compound-Poisson sum).  Deposits of one history that
-    fall in one block are correlated, so V is a slight underestimate; the
-    engine-against-engine comparison below measures that inflation."""
-    ex, vx, hx = _block(x["edep"], b), _block(x["edep2"], b), _block(x["hit"], b)
-    ey, vy, hy = _block(y["edep"], b), _block(y["edep2"], b), _block(y["hit"], b)
-    keep = (hx >= min_hits) & (hy >= min_hits)
-    return (ex[keep] - ey[keep]) / np.sqrt(vx[keep] + vy[keep])
-
-
-def _pixel_z(x, y, b=8, min_counts=20):
-    """z-scores of the detector counts in blocks of b x b pixels (at 10^7
-    primaries the 256 x 256 pixels hold about 0.3 counts each)."""
-    def blocks(h):
-        ny, nx = h.shape
-        return h.astype(np.float64).reshape(ny // b, b, nx // b, b).sum(axis=(1, 3))
-    hx, hy = blocks(x["hist"][0]), blocks(y["hist"][0])
-    keep = hx + hy >= min_counts
-    return (hx[keep] - hy[keep]) / np.sqrt(hx[keep] + hy[keep])
-
-
 def test_dose_and_projection_statistics_against_oracle(body):
     """Per-voxel agreement of the dose map and per-pixel agreement of the
     projection with independent seeds (z-score and chi-square checks of
@@ -123,14 +95,7 @@ def test_dose_and_projection_statistics_against_oracle(body):
     c = body.run_engine(n, 1003, eng)
     eng.close()
     ref = body.run_oracle(n, 1002, threads=0)
-    for name, fn, min_cells in (("dose blocks 8^3", _dose_z, 2000), ("projection 8x8 pixel blocks", _pixel_z, 300)):
-        z_ab, z_ac = fn(a, ref), fn(a, c)
-        m = z_ab.size
-        chi_ab, chi_ac = float(np.mean(z_ab ** 2)), float(np.mean(z_ac ** 2))
-        print("[stat] %s: %d cells, engine-oracle mean z %.4f chi2/dof %.4f; engine-engine mean z %.4f "
-              "chi2/dof %.4f" % (name, m, z_ab.mean(), chi_ab, z_ac.mean(), chi_ac), flush=True)
-        assert m > min_cells, (name, m)
-        se = np.sqrt(max(chi_ab, 1.0) / m)
-        assert abs(z_ab.mean()) < 6 * se, (name, z_ab.mean(), se)
-        assert chi_ac < 1.5, (name, chi_ac)
-        assert chi_ab < chi_ac + 6 * np.sqrt(2.0 / m) * max(chi_ac, 1.0), (name, chi_ab, chi_ac)
+    parity.same_distribution("dose blocks 8^3, engine vs oracle (null: engine vs engine)",
+                             parity.dose_z(a, ref), parity.dose_z(a, c), 2000)
+    parity.same_distribution("projection 8x8 pixel blocks, engine vs oracle (null: engine vs engine)",
+                             parity.pixel_z(a, ref), parity.pixel_z(a, c), 300)
