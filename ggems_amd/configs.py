@@ -345,6 +345,45 @@ def scatter_scene(workdir, phantom_seed=None, angle_deg=0.0, shape=(64, 96, 96),
     return keep
 
 
+def scatter_scenes(workdir, phantom_seeds, shape=(64, 96, 96), voxel=2.0, detector=DETECTOR):
+    """The scatter-surrogate scene with several seeded body phantoms, all at
+    the isocentre, for one acquisition launch in which each projection sees
+    one of them (GGEMS.run_acquisition with `phantoms`): the phantoms of
+    scatter_scene, registered in the order of phantom_seeds, then the
+    detector and the source.  Returns the objects that must stay alive."""
+    os.makedirs(workdir, exist_ok=True)
+    spec = os.path.join(workdir, "spectrum_120kVp_2.5mmAl.dat")
+    if not os.path.exists(spec):
+        GGEMSMaterialsDatabaseManager().set_materials(MATERIALS_TXT)
+        e, w = authoring.kramers_spectrum(120.0, 20.0, 2.5, authoring.aluminium_mu())
+        authoring.write_spectrum(spec, e, w)
+    nz, ny, nx = shape
+    mhds = []
+    for seed in phantom_seeds:
+        mhd = os.path.join(workdir, "body_%d_%dx%dx%d.mhd" % (seed, nx, ny, nz))
+        if not os.path.exists(mhd):
+            lab = authoring.ellipsoid_body_phantom(seed, shape=shape, voxel=voxel,
+                                                   body_axes=(0.45 * nx * voxel, 0.35 * ny * voxel),
+                                                   semi_axis_range=(nx * voxel / 50.0, nx * voxel / 5.0))
+            authoring.write_mhd(mhd, lab, (voxel, voxel, voxel))
+        mhds.append(mhd)
+    rng = authoring.write_range(os.path.join(workdir, "range_body.txt"), authoring.BODY_MATERIALS)
+    setup(workdir)
+    keep = []
+    for k, mhd in enumerate(mhds):
+        phantom = GGEMSVoxelizedPhantom("phantom_%d" % k)
+        phantom.set_phantom(mhd, rng)
+        phantom.set_rotation(0.0, 0.0, 0.0, "deg")
+        phantom.set_position(0.0, 0.0, 0.0, "mm")
+        keep.append(phantom)
+    keep.append(_detector(workdir, n=detector))
+    _common_physics()
+    src = _source(1, detector=detector)
+    src.set_polyenergy(spec)
+    keep.append(src)
+    return keep
+
+
 def run(seed=777):
     g = GGEMS()
     g.initialize(seed)

@@ -38,6 +38,7 @@
 #include <mutex>
 #include <random>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../../include/ggems_engine.h"
@@ -197,14 +198,27 @@ struct Lane {
   float tau;      // tau sampling: mean free paths left in the current flight
 };
 
-// Run counters of a wave, 64-bit in LDS rather than per lane in registers
-// (the kernel's scarce resource): a counting site adds the number of its
-// active lanes through the first of them.  Each wave owns its counters, and
-// a wave's LDS operations complete in program order, so no update is lost.
+// Run counters.  Which form is faster depends on the kernel variant's
+// register allocation (measured on the voxel-CT workload), so each variant
+// takes the faster one; both count exactly the same events.
 enum { kCntPrimaries = 0, kCntInteractions, kCntDetectorSteps, kCntDetectorInteractions, kCntHops, kNumCounters };
-struct Counters {
+
+// 32-bit counters per lane, summed per wave in 64 bits at the end (the
+// reference-sampling variants).  A lane wraps only past 2^32 counted events
+// in one launch: with runs capped at 2^48 primaries, spread over 2^18 lanes
+// by the slot queue, a lane sees about 2^30 primaries of a few events each.
+struct LaneCounters {
+  uint32_t v[kNumCounters];
+  __device__ inline void add(int k) { ++v[k]; }
+};
+
+// 64-bit counters per wave in LDS (the tau-sampling variants): a counting
+// site adds the number of its active lanes through the first of them.  Each
+// wave owns its counters and a wave's LDS operations complete in program
+// order, so no update is lost.
+struct WaveCounters {
   unsigned long long* w;  // this wave's kNumCounters counters in LDS
-  __device__ inline void add(int k) const {
+  __device__ inline void add(int k) {
     const uint64_t m = __ballot(1);
     if (__lane_id() == __ffsll((unsigned long long)m) - 1) w[k] += (unsigned long long)__popcll(m);
   }
@@ -564,8 +578,8 @@ __device__ inline bool walk_step(const VoxGeom& G, f3& p, const Ray& ray, int32_
 // ---------------------------------------------------------------------------
 // kAcq: the module sits where the lane's projection puts it and scores into
 // that projection's image.
-template <bool kAcq>
-__device__ void track_box(const RunParams& P, const Tabs& T, const DevSolid& s, Lane& L, const Counters& c) {
+template <bool kAcq, class Counters>
+__device__ void track_box(const RunParams& P, const Tabs& T, const DevSolid& s, Lane& L, Counters& c) {
   const Mat34& m = kAcq ? P.acq_box[L.proj * P.n_box + s.box_index] : s.m;
   f3 lp = g2l_pos(m, L.p);
   Ray ld;
@@ -856,11 +870,21 @@ __device__ int find_and_project(const RunParams& P, const Tabs& T, f3& pos, f3 d
 
 // a wave's counters into the run statistics (ggh_stats order: primaries,
 // voxel steps, interactions, detector steps, detector interactions, hops)
-__device__ inline void flush_wave_counters(const RunParams& P, const Counters& c) {
+__device__ inline int stat_slot(int k) { return k == kCntPrimaries ? 0 : k + 1; }
+
+__device__ inline void flush_counters(const RunParams& P, const LaneCounters& c) {
+  for (int k = 0; k < kNumCounters; ++k) {
+    unsigned long long t = c.v[k];
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+    if (__lane_id() == 0 && t) atomicAdd(&P.stats[stat_slot(k)], t);
+  }
+}
+
+__device__ inline void flush_counters(const RunParams& P, const WaveCounters& c) {
   const int k = __lane_id();
   if (k < kNumCounters) {
     const unsigned long long v = c.w[k];
-    if (v) atomicAdd(&P.stats[k == kCntPrimaries ? 0 : k + 1], v);
+    if (v) atomicAdd(&P.stats[stat_slot(k)], v);
   }
 }
 
@@ -882,8 +906,8 @@ __device__ inline void store_stream(const RunParams& P, uint32_t slot, const Rng
 
 // The discrete process a voxel step stopped on, then the energy cut
 // (TrackThroughGGEMSVoxelizedSolid.cl:238-310).
-template <int NP, bool kTau>
-__device__ inline void resolve_interaction(const RunParams& P, const Tabs& T, Lane& L, const Counters& c) {
+template <int NP, bool kTau, class Counters>
+__device__ inline void resolve_interaction(const RunParams& P, const Tabs& T, Lane& L, Counters& c) {
   const int gmat = L.pend >> 2;
   int proc = L.pend & 3;
   if (kTau) {
@@ -936,8 +960,8 @@ __device__ inline void next_projection(const RunParams& P, Lane& L) {
 
 // the cold work of one lane, one state at a time in an order that lets a
 // lane pass through several states in one call
-template <int NP, bool kSingle, bool kAcq, bool kTau>
-__device__ inline void cold_step(const RunParams& P, const Tabs& T, Lane& L, const Counters& c) {
+template <int NP, bool kSingle, bool kAcq, bool kTau, class Counters>
+__device__ inline void cold_step(const RunParams& P, const Tabs& T, Lane& L, Counters& c) {
   if (L.mode == kInteract) resolve_interaction<NP, kTau>(P, T, L, c);
   if (L.mode == kCut) {
     if (L.solid == P.dose_solid) dose_record(P, L.e, L.p);
@@ -1105,8 +1129,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWavesPe
 void transport_kernel(RunParams P) {
   extern __shared__ float4 s_tab[];
   __shared__ unsigned long long s_counts[kBlock / 64][kNumCounters];
-  if (threadIdx.x < (kBlock / 64) * kNumCounters) (&s_counts[0][0])[threadIdx.x] = 0ull;
-  __syncthreads();
+  if (kTau) {
+    if (threadIdx.x < (kBlock / 64) * kNumCounters) (&s_counts[0][0])[threadIdx.x] = 0ull;
+    __syncthreads();
+  }
   Tabs T = {P.tab_sig, P.tab_rcp, P.ebins, P.solids, P.spectrum, P.cdf};
   if (kLDS) {
     // LDS layout (the host sizes it with the same arithmetic, lds_layout):
@@ -1156,7 +1182,9 @@ void transport_kernel(RunParams P) {
   L.slot = 0xffffffffu;
   L.proj = 0;
   L.tau = 0.0f;
-  const Counters c = {s_counts[threadIdx.x >> 6]};
+  typename std::conditional<kTau, WaveCounters, LaneCounters>::type c;
+  if constexpr (kTau) c.w = s_counts[threadIdx.x >> 6];
+  else for (int k = 0; k < kNumCounters; ++k) c.v[k] = 0u;
   unsigned long long steps = 0;  // wave-uniform
 
   if (!kRegroup) {
@@ -1218,8 +1246,11 @@ void transport_kernel(RunParams P) {
         }
         steps += __popcll(__ballot(stepped));
         // pending interactions resolved in batches, so that scattered photons
-        // keep stepping without waiting for the next round
-        if (__popcll(__ballot(L.mode == kInteract)) >= P.interact_batch) {
+        // keep stepping without waiting for the next round.  Tau sampling
+        // leaves them to the navigation passes: its voxel loop runs faster
+        // without the interaction code (measured), and its steps are cheap
+        // enough that the waiting lanes cost less.
+        if (!kTau && __popcll(__ballot(L.mode == kInteract)) >= P.interact_batch) {
           if (L.mode == kInteract) resolve_interaction<NP, kTau>(P, T, L, c);
         }
       }
@@ -1254,7 +1285,7 @@ void transport_kernel(RunParams P) {
     }
   }
 
-  flush_wave_counters(P, c);
+  flush_counters(P, c);
   if (__lane_id() == 0 && steps) atomicAdd(&P.stats[1], steps);
 }
 
@@ -1899,6 +1930,8 @@ int check_ready(ggh_context* c) {
 // primaries: 16 batches, the first of 2^20 + 1 photons), whose largest batch
 // would overrun the particle buffer; such n take the exact ceiling instead.
 int batch_layout(uint64_t n, uint32_t& n_batches, uint32_t& base, uint32_t& rem, uint32_t& slots) {
+  // the bound under which no 32-bit lane run counter can wrap (LaneCounters)
+  if (n > (1ull << 48)) return fail(-2, "a run transports at most 2^48 primaries");
   uint64_t nb = (uint64_t)std::ceil((float)n / (float)GGH_MAX_SLOTS);
   if (nb == 0) nb = 1;
   if (nb * GGH_MAX_SLOTS < n) nb = (n + GGH_MAX_SLOTS - 1) / GGH_MAX_SLOTS;

@@ -315,6 +315,69 @@ def dataset(workdir, n_views, seed, n_low, n_high, rate, detector=256, shape=(64
                 np.array([v[k] for v in views])) for k in views[0]}
 
 
+def dataset_batched(workdir, phantom_seeds, angles_per_phantom, seed, n_low, n_high, rate, detector=256,
+                    shape=(64, 96, 96), log=None):
+    """Views of several seeded phantoms at seeded random gantry angles, all
+    from three acquisition launches (GGEMS.run_acquisition with one phantom
+    per projection): one of n_low primaries per view for the inputs, and the
+    two halves of n_high for the target and its variance.  The fields are
+    those of `make_view`; t_low, t_drr and t_high are each launch's wall
+    time divided by the number of views."""
+    import time
+    from . import configs
+    from .ggems import GGEMS
+    seeds = list(phantom_seeds)
+    keep = configs.scatter_scenes(workdir, seeds, shape=shape, detector=detector)
+    g = GGEMS()
+    g.initialize(seed)
+    eng = g.engine()
+    box = len(seeds)  # the detector follows the phantoms
+    eng.box_elements[box] = (detector, detector, True)
+    rng = np.random.default_rng(seed)
+    angles = rng.uniform(0.0, 360.0, size=(len(seeds), angles_per_phantom))
+    rot = np.zeros((angles.size, 3), np.float32)
+    rot[:, 2] = angles.ravel()
+    only = np.repeat(np.arange(len(seeds), dtype=np.int32), angles_per_phantom)
+    n_views = rot.shape[0]
+    norm = np.maximum(rate, 1e-12)
+
+    def acquire(n):
+        t0 = time.perf_counter()
+        g.run_acquisition(rot, np.full(n_views, n, np.uint64), "deg", only)
+        images = [eng.projection(box, p) for p in range(n_views)]
+        return (time.perf_counter() - t0) / n_views, np.stack([h for h, _ in images]), \
+            np.stack([s for _, s in images])
+
+    if log:
+        log("%d views of %d phantoms: %d-primary inputs" % (n_views, len(seeds), n_low))
+    t_low, low, low_scatter = acquire(n_low)
+    t0 = time.perf_counter()
+    drr = np.stack([eng.drr(box, p) for p in range(n_views)])
+    t_drr = (time.perf_counter() - t0) / n_views
+    half = n_high // 2
+    if log:
+        log("targets: two halves of %d primaries" % n_high)
+    t_a, _, scatter_a = acquire(half)
+    t_b, _, scatter_b = acquire(n_high - half)
+    g.delete()
+    del keep
+    ya = scatter_a / (norm * half)
+    yb = scatter_b / (norm * (n_high - half))
+    return {"x": np.stack([low / (norm * n_low), drr], axis=-1).astype(np.float32),
+            "y": ((scatter_a + scatter_b) / (norm * n_high)).astype(np.float32),
+            "low": (low_scatter / (norm * n_low)).astype(np.float32),
+            "var": ((ya - yb) ** 2 / 4.0).astype(np.float32),
+            "t_low": np.full(n_views, t_low), "t_drr": np.full(n_views, t_drr), "t_high": np.full(n_views, t_a + t_b)}
+
+
+def drr_baseline(x_train, y_train):
+    """The method without the network: scatter = k * ray-traced transmission,
+    k fitted by least squares on the training views"""
+    d = x_train[..., 1].astype(np.float64)
+    k = float((d * y_train).sum() / max((d * d).sum(), 1e-300))
+    return lambda x: (k * x[..., 1]).astype(np.float32)
+
+
 def scatter_errors(pred, y, var, low):
     """Errors relative to the mean target, raw and against the truth.
 
