@@ -47,8 +47,22 @@ def algorithmic_bytes_per_primary(stats):
     return s * 1.0 + i * 40.0 + d * 8.0 + 48.0, s, i, d
 
 
-def cpu_baseline(workdir, n_per_step, seed, target_s, threads):
-    """The CPU oracle on a bounded sample of particle slots of the same workload."""
+def cpu_model():
+    """the host CPU's model name (/proc/cpuinfo), or None"""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(workdir, n_per_step, seed, target_s, threads, min_primaries=10_000_000):
+    """The CPU oracle on a bounded sample of particle slots of the same
+    workload: consecutive chunks of slots until at least target_s seconds
+    and min_primaries primaries."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     from scene import voxel_ct_scene
 
@@ -56,7 +70,7 @@ def cpu_baseline(workdir, n_per_step, seed, target_s, threads):
     prepared = sc.prepare_oracle()  # tallies allocated and faulted in outside the timed region
     # consecutive chunks of particle slots until about target_s seconds of CPU work
     begin, chunk, done, elapsed = 0, 2048, 0, 0.0
-    while elapsed < target_s and begin + chunk <= 1048576:
+    while (elapsed < target_s or done < min_primaries) and begin + chunk <= 1048576:
         t0 = time.perf_counter()
         r = sc.run_oracle(n_per_step, seed, threads=threads, slots=(begin, begin + chunk), prepared=prepared)
         elapsed += time.perf_counter() - t0
@@ -64,7 +78,7 @@ def cpu_baseline(workdir, n_per_step, seed, target_s, threads):
         begin += chunk
         rate = done / elapsed
         per_slot = r["stats"]["primaries"] / chunk
-        want = (target_s - elapsed) * rate / per_slot
+        want = max((target_s - elapsed) * rate, min_primaries - done) / per_slot
         chunk = int(min(max(2048, want / 2 if want > 4096 else want + 1), 1048576 - begin))
         if chunk <= 0:
             break
@@ -72,6 +86,8 @@ def cpu_baseline(workdir, n_per_step, seed, target_s, threads):
         "value": done / elapsed,
         "unit": "photons/s",
         "cores": threads,
+        "nproc": os.cpu_count(),
+        "cpu_model": cpu_model(),
         "kind": "port",
         "sample": "CPU oracle (oracle/, C restatement of the reference kernels), particle slots 0..%d of a "
                   "%d-primary voxel-CT step: %d primaries in %.1f s" % (begin, n_per_step, done, elapsed),
@@ -131,36 +147,50 @@ def bench_mesh_ct(args):
 
 
 def bench_acquisition(args):
-    """360-projection acquisitions around the three ellipsoid phantoms, each
-    one captured launch; value = all primaries / summed transport time"""
+    """The batched CT acquisition: 360 one-degree projections around each of
+    the 128^3, 256^3 and 384^3 ellipsoid phantoms, 1,080 projections of
+    uneven budgets in ONE captured launch (each projection sees its own
+    phantom), launch shape chosen by the autotuner; value = primaries / the
+    launch's transport time"""
+    import numpy as np
+
     from ggems_amd import configs
     from ggems_amd.ggems import GGEMS
     lo, hi = int(1_000_000 * args.budget_scale), int(30_000_000 * args.budget_scale)
-    per = []
-    for k, side in enumerate((128, 256, 384)):
-        wd = os.path.join(REPO, "ggems_amd", "data", "_gen", "acquisition")
-        keep = configs.acquisition(wd, side=side, seed=1 + k)
-        rot, budgets = configs.acquisition_plan(n_angles=args.angles, budget_range=(lo, hi), seed=5 + k)
-        g = GGEMS()
-        g.initialize(args.seed + k)
-        g.autotune(4_000_000)
-        g.run_acquisition(rot, budgets, "deg")
-        st = g.engine_stats()
-        ms = g.run_time_ms()
-        n = int(budgets.sum())
-        assert st["primaries"] == n, st
-        per.append({"side": side, "primaries": n, "transport_ms": ms, "photons_per_s": n / ms * 1e3,
-                    "voxel_steps_per_primary": st["voxel_steps"] / n})
-        g.delete()
-        del keep
-    total = sum(p["primaries"] for p in per)
-    t = sum(p["transport_ms"] for p in per) * 1e-3
-    out = {"metric": "primary photons/s on 1 MI355X (CT acquisition, one launch per phantom)", "value": total / t,
+    sides = (128, 256, 384)
+    wd = os.path.join(REPO, "ggems_amd", "data", "_gen", "acquisition")
+    keep = configs.acquisition(wd, sides=sides, seeds=(1, 2, 3))
+    rot, budgets, only = [], [], []
+    for k in range(len(sides)):
+        r, b = configs.acquisition_plan(n_angles=args.angles, budget_range=(lo, hi), seed=5 + k)
+        rot.append(r)
+        budgets.append(b)
+        only += [k] * args.angles
+    rot, budgets = np.concatenate(rot), np.concatenate(budgets)
+    g = GGEMS()
+    g.initialize(args.seed)
+    eng = g.engine()
+    eng.set_sampling(args.sampling)
+    g.autotune(4_000_000)
+    g.run_acquisition(rot, budgets, "deg", only)
+    grid = list(eng.last_grid())
+    st = g.engine_stats()
+    ms = g.run_time_ms()
+    n = int(budgets.sum())
+    assert st["primaries"] == n, st
+    per = [{"side": s, "primaries": int(budgets[k * args.angles:(k + 1) * args.angles].sum())}
+           for k, s in enumerate(sides)]
+    g.delete()
+    del keep
+    out = {"metric": "primary photons/s on 1 MI355X (batched CT acquisition in one launch)", "value": n / ms * 1e3,
            "unit": "photons/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": "%d projections (1 degree) around seeded ellipsoid phantoms of 128^3, 256^3, "
-                                  "384^3 1 mm voxels, budgets uniform in [%d, %d], flat 512x384 GOS panel of "
-                                  "0.78 mm at SDD 1000 mm, 120 kVp" % (args.angles, lo, hi),
-                      "phantoms": per}}
+           "ms_per_step": ms,
+           "config": {"workload": "%d projections (1 degree) around each of three seeded ellipsoid phantoms of "
+                                  "128^3, 256^3, 384^3 1 mm voxels in one captured launch, budgets uniform in "
+                                  "[%d, %d], flat 512x384 GOS panel of 0.78 mm at SDD 1000 mm, 120 kVp"
+                                  % (args.angles, lo, hi),
+                      "sampling": args.sampling, "projections": int(rot.shape[0]), "primaries": n,
+                      "voxel_steps_per_primary": st["voxel_steps"] / n, "grid": grid, "phantoms": per}}
     print(json.dumps(out), flush=True)
 
 
@@ -181,22 +211,27 @@ def bench_surrogate(args):
 
     t0 = time.perf_counter()
     rate = S.open_field_rate(wd, args.n_high, seed=3)
-    tr = S.dataset(wd, args.train_views, 100, args.n_low, args.n_high, rate, log=log)
-    te = S.dataset(wd, args.test_views, 100000, args.n_low, args.n_eval, rate, log=log)
+    tr = S.dataset_chunks(wd, range(1, 1 + args.train_phantoms), args.train_angles, 100, args.n_low, args.n_high,
+                          rate, log=log)
+    te = S.dataset_chunks(wd, range(100001, 100001 + args.test_phantoms), args.test_angles, 100000, args.n_low,
+                          args.n_eval, rate, log=log)
+    n_train, n_test = tr["x"].shape[0], te["x"].shape[0]
     t_data = time.perf_counter() - t0
-    net, t_train, losses = S.train(tr["x"], tr["y"], steps=args.train_steps, batch=8, seed=1, log=log)
+    net, t_train, losses = S.train(tr["x"], tr["y"], steps=args.train_steps, batch=args.train_batch, seed=1,
+                                   log=log)
     S.predict(net, te["x"][:8])  # warm
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     pred = S.predict(net, te["x"])
-    t_infer = (time.perf_counter() - t0) / args.test_views
+    t_infer = (time.perf_counter() - t0) / n_test
     err = S.scatter_errors(pred, te["y"], te["var"], te["low"])
+    base = S.scatter_errors(S.drr_baseline(tr["x"], tr["y"])(te["x"]), te["y"], te["var"], te["low"])
     # wall time per view: the surrogate's inputs (low-count MC, ray trace) and
     # its inference, against plain MC with enough primaries to match its error
     t_sur = float(np.mean(te["t_low"] + te["t_drr"])) + t_infer
     mc_rate = args.n_eval / float(np.mean(te["t_high"]))
     t_mc = err["mc_primaries_factor"] * args.n_low / mc_rate
-    flops_step = 3 * 8 * S.conv_flops(net, 256, 256)
+    flops_step = 3 * args.train_batch * S.conv_flops(net, 256, 256)
     # the MFMA convolution on its own: the widest decoder layer, batch 8
     x = torch.randn(8, 64, 64, 192, device="cuda").bfloat16()
     w = torch.randn(64, 9 * 192, device="cuda").bfloat16()
@@ -214,15 +249,19 @@ def bench_surrogate(args):
            "value": err["pred"], "unit": "relative RMS", "n_gpus": 1, "higher_is_better": False,
            "dtype": "bf16", "data": "synthetic",
            "config": {"workload": "U-Net (4 levels, 16-32-64-128, 3x3 conv, BF16 MFMA, FP32 accumulate) on 256x256 "
-                                  "views of seeded 96x96x64 body phantoms (2 mm), 120 kVp; inputs: %d-primary MC "
-                                  "projection and the ray-traced transmission; training targets: %d-primary MC "
-                                  "scatter; held-out targets: %d-primary MC scatter" % (args.n_low, args.n_high,
-                                                                                        args.n_eval),
-                      "train_views": args.train_views, "test_views": args.test_views, "train_steps": args.train_steps},
+                                  "views of seeded 96x96x64 body phantoms (2 mm), 120 kVp, generated by batched "
+                                  "acquisitions (one launch per group of phantoms x gantry angles); inputs: "
+                                  "%d-primary MC projection (counts and scatter counts) and the ray-traced "
+                                  "transmission; training targets: "
+                                  "%d-primary MC scatter; held-out targets: %d-primary MC scatter"
+                                  % (args.n_low, args.n_high, args.n_eval),
+                      "train_views": n_train, "test_views": n_test, "train_steps": args.train_steps,
+                      "train_batch": args.train_batch},
            "low_count_mc_relative_rms": err["low"],
            "target_noise_relative_rms": err["target_noise"],
            "surrogate_vs_truth_relative_rms": err["pred_vs_truth"],
            "low_count_mc_vs_truth_relative_rms": err["low_vs_truth"],
+           "drr_scatter_model_vs_truth_relative_rms": base["pred_vs_truth"],
            "mc_primaries_for_equal_error": err["mc_primaries_factor"] * args.n_low,
            "seconds_per_view": {"surrogate": t_sur, "low_count_mc": float(np.mean(te["t_low"])),
                                 "ray_trace": float(np.mean(te["t_drr"])), "inference": t_infer,
@@ -247,7 +286,9 @@ def main():
     ap.add_argument("--seed", type=int, default=777)
     ap.add_argument("--workdir", default=os.path.join(REPO, "ggems_amd", "data", "_gen", "voxel_ct"))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    # the GPU box gives each GPU 16 host CPUs (OMP_NUM_THREADS there); elsewhere every core
+    ap.add_argument("--cpu-threads", type=int,
+                    default=int(os.environ.get("OMP_NUM_THREADS") or 0) or (os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--round-steps", type=int, default=0,
                     help="voxel steps a wave makes at least between regroupings (0: engine default)")
@@ -266,11 +307,14 @@ def main():
     ap.add_argument("--n-low", type=int, default=1_000_000, help="surrogate: primaries of the input projection")
     ap.add_argument("--n-high", type=int, default=100_000_000, help="surrogate: primaries of the target")
     ap.add_argument("--n-eval", type=int, default=1_000_000_000, help="surrogate: primaries of held-out targets")
-    ap.add_argument("--train-views", type=int, default=192)
-    ap.add_argument("--test-views", type=int, default=12)
+    ap.add_argument("--train-phantoms", type=int, default=8, help="surrogate: seeded training phantoms")
+    ap.add_argument("--train-angles", type=int, default=24, help="surrogate: gantry angles per training phantom")
+    ap.add_argument("--test-phantoms", type=int, default=4, help="surrogate: held-out phantoms")
+    ap.add_argument("--test-angles", type=int, default=3, help="surrogate: gantry angles per held-out phantom")
     ap.add_argument("--train-steps", type=int, default=8000)
+    ap.add_argument("--train-batch", type=int, default=8, help="surrogate: views per Adam step")
     ap.add_argument("--angles", type=int, default=360, help="acquisition: projections per phantom")
-    ap.add_argument("--budget-scale", type=float, default=0.1,
+    ap.add_argument("--budget-scale", type=float, default=1.0,
                     help="acquisition: budgets uniform in [1e6, 3e7] times this")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

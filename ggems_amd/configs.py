@@ -274,15 +274,24 @@ def acquisition_plan(n_angles=360, budget_range=(1_000_000, 30_000_000), seed=5,
     return r, budgets
 
 
-def acquisition(workdir, side=128, seed=1):
-    """The acquisition scene (phantom, flat panel, source); transport with
-    GGEMS().run_acquisition(rotations, budgets) after initialize."""
-    mhd, rng, spec = acquisition_inputs(workdir, side, seed)
-    setup(workdir)
-    phantom = GGEMSVoxelizedPhantom("phantom")
-    phantom.set_phantom(mhd, rng)
-    phantom.set_rotation(0.0, 0.0, 0.0, "deg")
-    phantom.set_position(0.0, 0.0, 0.0, "mm")
+def acquisition(workdir, side=128, seed=1, sides=None, seeds=None):
+    """The acquisition scene (phantoms, flat panel, source); transport with
+    GGEMS().run_acquisition(rotations, budgets) after initialize.  With
+    `sides` (and `seeds`, one per side), one ellipsoid phantom per side, all
+    at the isocentre, for an acquisition whose projections each see one of
+    them (run_acquisition's `phantoms`, in this order)."""
+    if sides is None:
+        sides, seeds = (side,), (seed,)
+    keep = []
+    for k, (sd, sdseed) in enumerate(zip(sides, seeds)):
+        mhd, rng, spec = acquisition_inputs(workdir, sd, sdseed)
+        if k == 0:
+            setup(workdir)
+        phantom = GGEMSVoxelizedPhantom("phantom" if len(sides) == 1 else "phantom_%d" % k)
+        phantom.set_phantom(mhd, rng)
+        phantom.set_rotation(0.0, 0.0, 0.0, "deg")
+        phantom.set_position(0.0, 0.0, 0.0, "mm")
+        keep.append(phantom)
     ct = GGEMSCTSystem("detector")
     ct.set_ct_type("flat")
     ct.set_number_of_modules(1, 1)
@@ -305,7 +314,7 @@ def acquisition(workdir, side=128, seed=1):
     src.set_beam_aperture(math.degrees(math.atan(half_diag / SDD)), "deg")
     src.set_focal_spot_size(0.0, 0.0, 0.0, "mm")
     src.set_polyenergy(spec)
-    return [phantom, ct, src]
+    return keep + [ct, src]
 
 
 def scatter_scene(workdir, phantom_seed=None, angle_deg=0.0, shape=(64, 96, 96), voxel=2.0, detector=DETECTOR):

@@ -1211,7 +1211,7 @@ void transport_kernel(RunParams P) {
     // Every wave passes the same barriers in every round, and every loop
     // above is bounded, so the block cannot deadlock.
     uint4* s_x = reinterpret_cast<uint4*>(reinterpret_cast<char*>(s_tab) + P.xchg_offset);
-    __shared__ uint32_t s_cnt[kWaves][2];
+    __shared__ uint32_t s_cnt[kWaves][kTau ? 3 : 2];
     __shared__ uint32_t s_cold_done[2];
     __shared__ uint32_t s_ready[2];
     const int wave = (int)(threadIdx.x >> 6), lane = __lane_id();
@@ -1254,34 +1254,74 @@ void transport_kernel(RunParams P) {
           if (L.mode == kInteract) resolve_interaction<NP, kTau>(P, T, L, c);
         }
       }
-      // regrouping: rank 0 inside a phantom, 1 stopped, 2 no work left
-      const int key = L.mode == kInVoxel ? 0 : (L.mode != kDone ? 1 : 2);
-      const uint64_t b0 = __ballot(key == 0), b1 = __ballot(key == 1);
-      if (lane == 0) {
-        s_cnt[wave][0] = (uint32_t)__popcll(b0);
-        s_cnt[wave][1] = (uint32_t)__popcll(b1);
-      }
-      __syncthreads();
-      uint32_t n_hot = 0, n_cold = 0, before0 = 0, before1 = 0, before2 = 0;
-      for (int w = 0; w < kWaves; ++w) {
-        const uint32_t h = s_cnt[w][0], cl = s_cnt[w][1];
-        n_hot += h;
-        n_cold += cl;
-        if (w < wave) {
-          before0 += h;
-          before1 += cl;
-          before2 += 64u - h - cl;
+      if constexpr (kTau) {
+        // regrouping by event: rank 0 inside a phantom, 1 a pending
+        // interaction, 2 other navigation (exit, cut, new primary or slot),
+        // 3 no work left, so that the next navigation passes run one kind of
+        // event per wave (measured faster for tau sampling, whose
+        // interactions all wait for these passes; slower for the reference
+        // sampling, which resolves most of them inside the voxel loop)
+        const int key = L.mode == kInVoxel ? 0 : (L.mode == kInteract ? 1 : (L.mode != kDone ? 2 : 3));
+        const uint64_t b0 = __ballot(key == 0), b1 = __ballot(key == 1), b2 = __ballot(key == 2);
+        if (lane == 0) {
+          s_cnt[wave][0] = (uint32_t)__popcll(b0);
+          s_cnt[wave][1] = (uint32_t)__popcll(b1);
+          s_cnt[wave][2] = (uint32_t)__popcll(b2);
         }
+        __syncthreads();
+        uint32_t n0 = 0, n1 = 0, n2 = 0, before0 = 0, before1 = 0, before2 = 0, before3 = 0;
+        for (int w = 0; w < kWaves; ++w) {
+          const uint32_t c0 = s_cnt[w][0], c1 = s_cnt[w][1], c2 = s_cnt[w][2];
+          n0 += c0;
+          n1 += c1;
+          n2 += c2;
+          if (w < wave) {
+            before0 += c0;
+            before1 += c1;
+            before2 += c2;
+            before3 += 64u - c0 - c1 - c2;
+          }
+        }
+        if (n0 + n1 + n2 == 0) break;  // read from LDS after a barrier: the same for the whole block
+        if (threadIdx.x == 0) {  // the next round's counters; nobody reads them before the barriers below
+          s_cold_done[rb ^ 1u] = 0;
+          s_ready[rb ^ 1u] = 0;
+        }
+        const uint64_t mine = key == 0 ? b0 : (key == 1 ? b1 : (key == 2 ? b2 : ~(b0 | b1 | b2)));
+        const uint32_t pos = (key == 0 ? before0 : (key == 1 ? n0 + before1 : (key == 2 ? n0 + n1 + before2
+                                                                                 : n0 + n1 + n2 + before3))) +
+                             (uint32_t)__popcll(mine & below);
+        exchange_lane<kTau>(L, s_x, pos);
+      } else {
+        // regrouping: rank 0 inside a phantom, 1 stopped, 2 no work left
+        const int key = L.mode == kInVoxel ? 0 : (L.mode != kDone ? 1 : 2);
+        const uint64_t b0 = __ballot(key == 0), b1 = __ballot(key == 1);
+        if (lane == 0) {
+          s_cnt[wave][0] = (uint32_t)__popcll(b0);
+          s_cnt[wave][1] = (uint32_t)__popcll(b1);
+        }
+        __syncthreads();
+        uint32_t n_hot = 0, n_cold = 0, before0 = 0, before1 = 0, before2 = 0;
+        for (int w = 0; w < kWaves; ++w) {
+          const uint32_t h = s_cnt[w][0], cl = s_cnt[w][1];
+          n_hot += h;
+          n_cold += cl;
+          if (w < wave) {
+            before0 += h;
+            before1 += cl;
+            before2 += 64u - h - cl;
+          }
+        }
+        if (n_hot + n_cold == 0) break;  // read from LDS after a barrier: the same for the whole block
+        if (threadIdx.x == 0) {  // the next round's counters; nobody reads them before the barriers below
+          s_cold_done[rb ^ 1u] = 0;
+          s_ready[rb ^ 1u] = 0;
+        }
+        const uint64_t mine = key == 0 ? b0 : (key == 1 ? b1 : ~(b0 | b1));
+        const uint32_t pos = (key == 0 ? before0 : (key == 1 ? n_hot + before1 : n_hot + n_cold + before2)) +
+                             (uint32_t)__popcll(mine & below);
+        exchange_lane<kTau>(L, s_x, pos);
       }
-      if (n_hot + n_cold == 0) break;  // read from LDS after a barrier: the same for the whole block
-      if (threadIdx.x == 0) {  // the next round's counters; nobody reads them before the barriers below
-        s_cold_done[rb ^ 1u] = 0;
-        s_ready[rb ^ 1u] = 0;
-      }
-      const uint64_t mine = key == 0 ? b0 : (key == 1 ? b1 : ~(b0 | b1));
-      const uint32_t pos = (key == 0 ? before0 : (key == 1 ? n_hot + before1 : n_hot + n_cold + before2)) +
-                           (uint32_t)__popcll(mine & below);
-      exchange_lane<kTau>(L, s_x, pos);
     }
   }
 

@@ -578,6 +578,13 @@ class GGEMS:
     def repeat_acquisition(self):
         _call("repeat_acquisition_ggems", None, [VP], self.obj)
 
+    def set_sampling(self, mode):
+        """How voxel steps sample interactions (an addition with no GGEMS
+        counterpart; call after initialize): "reference", GGEMS's three
+        competing exponentials per step (the default), or "tau", one
+        exponential number of mean free paths per flight"""
+        self.engine().set_sampling(mode)
+
     def autotune(self, n_trial):
         _call("autotune_ggems", None, [VP, C.c_uint64], self.obj, int(n_trial))
 

@@ -12,11 +12,12 @@ level, 2x2 max-pooling down, nearest-neighbour 2x upsampling, concatenation
 with the skip, two 3x3 conv + ReLU per decoder level, and a 1x1 head.
 
 The training pairs come from the engine itself: for a seeded phantom and
-gantry angle, a projection of n_low primaries (counts over the open-field
-counts) and the ray-traced transmission are the inputs; the scatter image
-of n_high primaries, per primary and per unit open-field count, is the
-target.  The scatter image is the reference's scatter histogram
-(src/kernels/TrackThroughGGEMSSolidBox.cl:215-225).
+gantry angle, a projection of n_low primaries (its counts and its scatter
+counts, each over the open-field counts) and the ray-traced transmission are
+the inputs; the scatter image of n_high primaries, per primary and per unit
+open-field count, is the target.  The scatter image is the reference's
+scatter histogram (src/kernels/TrackThroughGGEMSSolidBox.cl:215-225), which
+the CT system stores beside the counts.
 """
 
 import ctypes as C
@@ -180,7 +181,7 @@ class UNet(torch.nn.Module):
         self.head_bias = torch.nn.Parameter(torch.zeros(1))
 
     def forward(self, x):
-        """x [N, H, W, 2] (any float type) -> [N, H, W] FP32; H, W multiples of 256 / 8"""
+        """x [N, H, W, k <= 8] (any float type) -> [N, H, W] FP32; H, W multiples of 32"""
         n, h, w, k = x.shape
         z = torch.zeros((n, h, w, IN_CHANNELS), dtype=torch.bfloat16, device=x.device)
         z[..., :k] = x.to(torch.bfloat16)
@@ -247,82 +248,23 @@ def open_field_rate(workdir, n, seed=1, detector=256):
     return counts.astype(np.float64) / n
 
 
-def _run_chunks(eng, n, chunk):
-    """n primaries in launches of at most `chunk`; the tallies accumulate"""
-    done = 0
-    while done < n:
-        m = min(chunk, n - done)
-        eng.run(m)
-        done += m
-
-
-def make_view(workdir, phantom_seed, angle_deg, n_low, n_high, rate, seed, detector=256, shape=(64, 96, 96),
-              chunk=100_000_000):
-    """One view of a seeded phantom, as a dict:
-
-    * ``x`` [H, W, 2]: low-count counts / open-field counts, and the
-      ray-traced transmission (the network's inputs);
-    * ``y`` [H, W]: high-count scatter / open-field counts (the target);
-    * ``low`` [H, W]: the low-count run's own scatter image, same scale;
-    * ``var`` [H, W]: an unbiased estimate of the target's variance, from
-      the two independent halves of the high-count run:
-      var(y) = E[(y_a - y_b)^2] / 4 for equal halves;
-    * ``t_low``, ``t_drr``, ``t_high``: wall seconds of the low-count run,
-      of the ray trace and of the high-count run (each with its read-back).
-    """
-    import time
-    from . import configs
-    keep = configs.scatter_scene(workdir, phantom_seed, angle_deg, shape=shape, detector=detector)
-    g, eng = _view_engine(1, seed, detector)
-    norm = np.maximum(rate, 1e-12)
-    t0 = time.perf_counter()
-    eng.run(n_low)
-    low, low_scatter = eng.histogram(1)
-    t_low = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    drr = eng.drr(1)
-    t_drr = time.perf_counter() - t0
-    eng.reset_tallies()
-    half = n_high // 2
-    t0 = time.perf_counter()
-    _run_chunks(eng, half, chunk)
-    _, scatter_a = eng.histogram(1)
-    _run_chunks(eng, n_high - half, chunk)
-    _, scatter = eng.histogram(1)
-    t_high = time.perf_counter() - t0
-    g.delete()
-    del keep
-    ya = scatter_a / (norm * half)
-    yb = (scatter - scatter_a) / (norm * (n_high - half))
-    return {"x": np.stack([low / (norm * n_low), drr], axis=-1).astype(np.float32),
-            "y": (scatter / (norm * n_high)).astype(np.float32),
-            "low": (low_scatter / (norm * n_low)).astype(np.float32),
-            "var": ((ya - yb) ** 2 / 4.0).astype(np.float32),
-            "t_low": t_low, "t_drr": t_drr, "t_high": t_high}
-
-
-def dataset(workdir, n_views, seed, n_low, n_high, rate, detector=256, shape=(64, 96, 96), log=None):
-    """n_views views of seeded phantoms (seed + k) at seeded random angles:
-    the fields of `make_view`, stacked (arrays) or listed (timings)"""
-    rng = np.random.default_rng(seed)
-    views = []
-    for k in range(n_views):
-        if log and k % 8 == 0:
-            log("view %d / %d" % (k, n_views))
-        views.append(make_view(workdir, seed + k, float(rng.uniform(0.0, 360.0)), n_low, n_high, rate,
-                               seed=1000 + seed + k, detector=detector, shape=shape))
-    return {k: (np.stack([v[k] for v in views]) if isinstance(views[0][k], np.ndarray) else
-                np.array([v[k] for v in views])) for k in views[0]}
-
-
 def dataset_batched(workdir, phantom_seeds, angles_per_phantom, seed, n_low, n_high, rate, detector=256,
                     shape=(64, 96, 96), log=None):
     """Views of several seeded phantoms at seeded random gantry angles, all
     from three acquisition launches (GGEMS.run_acquisition with one phantom
     per projection): one of n_low primaries per view for the inputs, and the
-    two halves of n_high for the target and its variance.  The fields are
-    those of `make_view`; t_low, t_drr and t_high are each launch's wall
-    time divided by the number of views."""
+    two halves of n_high for the target and its variance.  Returns a dict:
+
+    * ``x`` [V, H, W, 3]: the low-count projection's counts and scatter
+      counts / open-field counts, and the ray-traced transmission (the
+      network's inputs);
+    * ``y`` [V, H, W]: high-count scatter / open-field counts (the target);
+    * ``low`` [V, H, W]: the low-count run's own scatter image, same scale;
+    * ``var`` [V, H, W]: an unbiased estimate of the target's variance from
+      its two independent halves: var(y) = E[(y_a - y_b)^2] / 4;
+    * ``t_low``, ``t_drr``, ``t_high`` [V]: wall seconds per view of the
+      low-count launch, of the ray traces and of the two high-count
+      launches (each launch's time divided by the number of views)."""
     import time
     from . import configs
     from .ggems import GGEMS
@@ -363,11 +305,23 @@ def dataset_batched(workdir, phantom_seeds, angles_per_phantom, seed, n_low, n_h
     del keep
     ya = scatter_a / (norm * half)
     yb = scatter_b / (norm * (n_high - half))
-    return {"x": np.stack([low / (norm * n_low), drr], axis=-1).astype(np.float32),
+    return {"x": np.stack([low / (norm * n_low), drr, low_scatter / (norm * n_low)], axis=-1).astype(np.float32),
             "y": ((scatter_a + scatter_b) / (norm * n_high)).astype(np.float32),
             "low": (low_scatter / (norm * n_low)).astype(np.float32),
             "var": ((ya - yb) ** 2 / 4.0).astype(np.float32),
             "t_low": np.full(n_views, t_low), "t_drr": np.full(n_views, t_drr), "t_high": np.full(n_views, t_a + t_b)}
+
+
+def dataset_chunks(workdir, phantom_seeds, angles_per_phantom, seed, n_low, n_high, rate, chunk=5, detector=256,
+                   log=None):
+    """dataset_batched over groups of `chunk` phantoms (one GGEMS scene and
+    three launches per group), concatenated"""
+    seeds = list(phantom_seeds)
+    parts = []
+    for i in range(0, len(seeds), chunk):
+        parts.append(dataset_batched(workdir, seeds[i:i + chunk], angles_per_phantom, seed + i, n_low, n_high, rate,
+                                     detector=detector, log=log))
+    return {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
 
 
 def drr_baseline(x_train, y_train):

@@ -8,7 +8,9 @@ gantry angles) with 2*10^7-primary scatter targets, and 16 held-out views of
 independent halves so that its noise can be subtracted.  The U-Net trains
 for 1,500 Adam steps on the MFMA convolutions.  On the held-out views its
 error against the noise-corrected truth must beat both
-- the low-count (10^6-primary) Monte Carlo's own scatter image, and
+- the low-count (10^6-primary) Monte Carlo's own scatter image (one of the
+  network's three inputs, with that run's counts and the ray-traced
+  transmission), and
 - the method without the network: scatter = a constant times the ray-traced
   transmission, the constant fitted by least squares on the training views.
 """
@@ -31,7 +33,7 @@ def test_surrogate_beats_low_count_mc_and_the_drr_baseline():
     rate = S.open_field_rate(WORKDIR, 20_000_000, seed=3, detector=DET)
     tr = S.dataset_batched(WORKDIR, range(1, 9), 8, 100, 1_000_000, 20_000_000, rate, detector=DET)
     te = S.dataset_batched(WORKDIR, range(101, 105), 4, 200, 1_000_000, 200_000_000, rate, detector=DET)
-    assert tr["x"].shape == (64, DET, DET, 2) and te["y"].shape == (16, DET, DET)
+    assert tr["x"].shape == (64, DET, DET, 3) and te["y"].shape == (16, DET, DET)
     net, seconds, losses = S.train(tr["x"], tr["y"], steps=1500, batch=8, seed=1)
     assert losses[-1] < 0.5 * losses[0]
     pred = S.predict(net, te["x"])
