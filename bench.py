@@ -94,14 +94,16 @@ def cpu_baseline(workdir, n_per_step, seed, target_s, threads, min_primaries=10_
     }
 
 
-def profile_record():
-    """The committed PMC summary of the transport kernel, if it was measured on this engine source."""
+def profile_record(sampling):
+    """The committed PMC summary of the transport kernel, if it was measured on this engine source and
+    sampling mode."""
     path = os.path.join(REPO, "profiles", "transport_kernel_voxel_ct.json")
     try:
         with open(path) as f:
             rec = json.load(f)
         from ggems_amd.profiling import source_digest
-        return rec if rec.get("source_sha256") == source_digest(REPO) else None
+        ok = rec.get("source_sha256") == source_digest(REPO) and rec.get("sampling", "reference") == sampling
+        return rec if ok else None
     except (OSError, ValueError, KeyError):
         return None
 
@@ -298,9 +300,12 @@ def main():
                     help="navigation passes per round for stopped photons (0: engine default)")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--general-kernel", action="store_true", help="run the general kernel variant")
-    ap.add_argument("--sampling", choices=("reference", "tau"), default="reference",
-                    help="voxel-step sampling: GGEMS's three exponentials per step (reference) or one "
-                         "exponential number of mean free paths per flight (tau)")
+    ap.add_argument("--sampling", choices=("reference", "tau"), default="tau",
+                    help="voxel-step sampling: one exponential number of mean free paths per flight (tau, "
+                         "statistically equivalent to GGEMS's, the default) or GGEMS's three exponentials per "
+                         "step (reference, GGEMS's histories)")
+    ap.add_argument("--no-compare-sampling", dest="compare_sampling", action="store_false",
+                    help="skip timing the other sampling mode after the headline")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run a plain device copy of this size (calibrates profiler byte counters)")
     ap.add_argument("--workload", choices=("voxel_ct", "mesh_ct", "acquisition", "surrogate"), default="voxel_ct")
@@ -312,7 +317,7 @@ def main():
     ap.add_argument("--test-phantoms", type=int, default=4, help="surrogate: held-out phantoms")
     ap.add_argument("--test-angles", type=int, default=3, help="surrogate: gantry angles per held-out phantom")
     ap.add_argument("--train-steps", type=int, default=8000)
-    ap.add_argument("--train-batch", type=int, default=8, help="surrogate: views per Adam step")
+    ap.add_argument("--train-batch", type=int, default=16, help="surrogate: views per Adam step")
     ap.add_argument("--angles", type=int, default=360, help="acquisition: projections per phantom")
     ap.add_argument("--budget-scale", type=float, default=1.0,
                     help="acquisition: budgets uniform in [1e6, 3e7] times this")
@@ -365,10 +370,29 @@ def main():
     total = args.steps * args.photons_per_step
     assert stats["primaries"] == total, stats
     value = total / wall
+    other = None
+    if args.compare_sampling:
+        # the other sampling mode on the same scene, after the timed region
+        mode = "reference" if args.sampling == "tau" else "tau"
+        eng.set_sampling(mode)
+        eng.run(args.photons_per_step)
+        eng.reset_tallies()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        other_ms = []
+        for _ in range(args.steps):
+            eng.run(args.photons_per_step)
+            other_ms.append(eng.last_kernel_ms())
+        torch.cuda.synchronize()
+        other_wall = time.perf_counter() - t1
+        ost = eng.stats()
+        other = {"sampling": mode, "value": total / other_wall, "ms_per_step": other_wall / args.steps * 1e3,
+                 "kernel_ms": sum(other_ms) / len(other_ms), "voxel_steps_per_primary": ost["voxel_steps"] / total,
+                 "interactions_per_primary": ost["interactions"] / total}
     bpp, s, i, d = algorithmic_bytes_per_primary(stats)
     avg_kernel_s = sum(kernel_ms) / len(kernel_ms) * 1e-3
     achieved = bpp * args.photons_per_step / avg_kernel_s / 1e9
-    prof = profile_record()
+    prof = profile_record(args.sampling)
     traffic = prof["traffic_bytes_per_primary"] * args.photons_per_step if prof and prof.get(
         "traffic_bytes_per_primary") else None
     steps_per_launch = stats["voxel_steps"] / args.steps
@@ -409,6 +433,8 @@ def main():
             "bytes_per_primary": bpp,
         },
     }
+    if other:
+        out["other_sampling"] = other
     if prof and prof.get("valu_instr_per_step"):
         # issue-rate roofline of the same kernel (VALU wave instructions per voxel
         # step from the committed SQ counters of this source)

@@ -377,6 +377,22 @@ __global__ __launch_bounds__(kMaskBlock) void relu_bias_backward_kernel(const vo
   for (int i = threadIdx.x; i < C; i += kMaskBlock) unsafeAtomicAdd(&db[i], s_db[i]);
 }
 
+// The BF16 operands of a layer's forward and data-gradient convolutions from
+// its FP32 master weights, in one pass: w [cout][9][cin] rounded to nearest
+// even, and the data gradient's weights [cin][9][cout] with the taps
+// reversed (tap 8 - t), in place of a cast, a flip and a transposing copy.
+constexpr int kPrepBlock = 256;
+__global__ __launch_bounds__(kPrepBlock) void weights_bf16_kernel(const float* __restrict__ w, __bf16* __restrict__ wb,
+                                                                   __bf16* __restrict__ wt, int cout, int cin) {
+  const int n = cout * 9 * cin;
+  for (int i = blockIdx.x * kPrepBlock + threadIdx.x; i < n; i += gridDim.x * kPrepBlock) {
+    const int ci = i % cin, t = (i / cin) % 9, co = i / (9 * cin);
+    const __bf16 v = (__bf16)w[i];
+    wb[i] = v;
+    wt[((size_t)ci * 9 + (8 - t)) * cout + co] = v;
+  }
+}
+
 bool shapes_ok(int N, int H, int W, int Cin, int Cout) {
   return N > 0 && H > 0 && W > 0 && W % 16 == 0 && ((long long)N * H * W) % kFwdPix == 0 && Cin > 0 &&
          Cin % 8 == 0 && Cout > 0 && Cout % 16 == 0 && (long long)N * H * W * (Cin > Cout ? Cin : Cout) < (1ll << 31);
@@ -437,6 +453,18 @@ int ggh_relu_bias_backward(const void* dy, int32_t dy_is_f32, const void* y, int
                        static_cast<const __bf16*>(y), relu, static_cast<__bf16*>(dy_out), db, pieces, c);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return ufail(-100 - (int)e, std::string("ReLU / bias backward: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int ggh_conv3x3_weights_bf16(const float* w, void* w_bf16, void* wt_bf16, int32_t cout, int32_t cin, void* stream) {
+  if (!w || !w_bf16 || !wt_bf16) return ufail(-1, "null pointer");
+  if (cout < 1 || cin < 1 || (long long)cout * 9 * cin >= (1ll << 31)) return ufail(-2, "unsupported shape");
+  const long long n = (long long)cout * 9 * cin;
+  const unsigned blocks = (unsigned)((n + kPrepBlock - 1) / kPrepBlock < 4096 ? (n + kPrepBlock - 1) / kPrepBlock : 4096);
+  hipLaunchKernelGGL(weights_bf16_kernel, dim3(blocks), dim3(kPrepBlock), 0, static_cast<hipStream_t>(stream), w,
+                     static_cast<__bf16*>(w_bf16), static_cast<__bf16*>(wt_bf16), cout, cin);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return ufail(-100 - (int)e, std::string("BF16 weights: ") + hipGetErrorString(e));
   return 0;
 }
 

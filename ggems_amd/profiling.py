@@ -194,6 +194,7 @@ def main():
     ap.add_argument("--out")
     ap.add_argument("--json")
     ap.add_argument("--command", action="append", default=[])
+    ap.add_argument("--sampling", default="reference", help="the transport's sampling mode in the measured run")
     args = ap.parse_args()
     s = summarize(args.kt, args.fetch, args.write, args.sq, args.copy_mb, args.photons, args.kernel, args.extra,
                   args.steps, args.top)
@@ -208,7 +209,7 @@ def main():
         rec = {"kernel": s["kernel"], "photons": s["photons"], "traffic_bytes": s["traffic_bytes"],
                "traffic_bytes_per_primary": s.get("traffic_bytes_per_primary"),
                "fetch_factor": s["fetch_factor"], "write_factor": s["write_factor"],
-               "avg_ms": s["avg_ns"] * 1e-6, "source_sha256": source_digest()}
+               "avg_ms": s["avg_ns"] * 1e-6, "source_sha256": source_digest(), "sampling": args.sampling}
         c = s.get("sq", {})
         if args.steps and "SQ_INSTS_VALU" in c:
             rec["valu_instr_per_step"] = c["SQ_INSTS_VALU"] / args.steps

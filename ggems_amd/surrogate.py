@@ -35,6 +35,7 @@ _SIGS = {
     "ggh_conv3x3_weight_grad": ([C.c_void_p, C.c_void_p, C.c_void_p] + [C.c_int32] * 6 + [C.c_void_p], C.c_int),
     "ggh_relu_bias_backward": ([C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64,
                                 C.c_int32, C.c_void_p], C.c_int),
+    "ggh_conv3x3_weights_bf16": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p], C.c_int),
 }
 
 
@@ -113,24 +114,36 @@ def flip_transpose(w):
     return w.flip(1).permute(2, 1, 0).contiguous()
 
 
+def weights_bf16(w):
+    """(w as BF16 [cout][9][cin], flip_transpose of it [cin][9][cout]) from the
+    FP32 weights in one kernel"""
+    cout, _, cin = w.shape
+    w = w.detach().contiguous().float()
+    wb = torch.empty((cout, 9, cin), dtype=torch.bfloat16, device=w.device)
+    wt = torch.empty((cin, 9, cout), dtype=torch.bfloat16, device=w.device)
+    _check(_lib().ggh_conv3x3_weights_bf16(w.data_ptr(), wb.data_ptr(), wt.data_ptr(), cout, cin, _stream(w)))
+    return wb, wt
+
+
 class _Conv3x3(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x, w, b, relu):
         cout, _, cin = w.shape
-        wb = w.to(torch.bfloat16).contiguous()
+        wb, wt = weights_bf16(w)
         y = conv3x3(x, wb.view(cout, 9 * cin), b, relu)
-        ctx.save_for_backward(x, wb, y)
+        ctx.save_for_backward(x, wt, y)
         ctx.relu = relu
+        ctx.shape = (cout, cin)
         return y
 
     @staticmethod
     def backward(ctx, dy):
-        x, wb, y = ctx.saved_tensors
-        cout, _, cin = wb.shape
+        x, wt, y = ctx.saved_tensors
+        cout, cin = ctx.shape
         dy, db = relu_bias_backward(dy, y, ctx.relu)
         dx = None
         if ctx.needs_input_grad[0]:
-            dx = conv3x3(dy, flip_transpose(wb).view(cin, 9 * cout))
+            dx = conv3x3(dy, wt.view(cin, 9 * cout))
         dw = conv3x3_weight_grad(x, dy).view(cout, 9, cin)
         return dx, dw, db, None
 

@@ -307,6 +307,11 @@ int ggh_conv3x3_forward(const void* x, const void* w, const float* bias, void* y
 int ggh_conv3x3_weight_grad(const void* x, const void* dy, float* dw, int32_t n, int32_t h, int32_t w_px, int32_t cin,
                             int32_t cout, int32_t splits, void* stream);
 
+/* a layer's BF16 operands from its FP32 weights w [cout][9][cin]: w_bf16
+ * [cout][9][cin] (round to nearest even) and, for the data gradient, wt_bf16
+ * [cin][9][cout] with the taps reversed (tap 8 - t) */
+int ggh_conv3x3_weights_bf16(const float* w, void* w_bf16, void* wt_bf16, int32_t cout, int32_t cin, void* stream);
+
 /* backward of a layer's ReLU and bias: dy_out = dy where y > 0 (everywhere
  * when relu == 0), BF16; db[c] += sum over the pixels of dy_out, FP32.
  * dy is BF16, or FP32 when dy_is_f32; all NHWC with c one of 8, 16, 32, 64, 128 */

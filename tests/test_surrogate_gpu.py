@@ -104,3 +104,14 @@ def test_training_steps_reduce_the_loss():
         opt.step()
         first = first if first is not None else loss.item()
     assert loss.item() < 0.3 * first, (first, loss.item())
+
+
+@pytest.mark.parametrize("cout,cin", [(16, 8), (64, 192), (128, 128)])
+def test_weights_bf16_equal_cast_and_flip_transpose(cout, cin):
+    """the fused weight preparation equals torch's round-to-nearest cast and
+    the data gradient's flipped, transposed weights, bit for bit"""
+    w = torch.randn(cout, 9, cin, device="cuda") * 0.1
+    wb, wt = S.weights_bf16(w)
+    ref = w.to(torch.bfloat16)
+    assert torch.equal(wb.view(torch.int16), ref.view(torch.int16))
+    assert torch.equal(wt.view(torch.int16), S.flip_transpose(ref).view(torch.int16))
