@@ -1113,6 +1113,17 @@ __device__ inline void exchange_lane(Lane& L, uint4* s_x, uint32_t pos) {
   __syncthreads();
 }
 
+// The kernel's RunParams argument, read through a pointer the compiler cannot
+// follow past the empty asm: fields the navigation code reads through it are
+// loaded (scalar loads from the kernel argument segment, RunParams being the
+// first argument) where that code runs, instead of being hoisted to the
+// kernel's start and kept in scalar registers across the voxel loop.
+__device__ inline const RunParams& cold_params() {
+  auto k = __builtin_amdgcn_kernarg_segment_ptr();
+  asm volatile("" : "+s"(k));
+  return *(const RunParams*)k;
+}
+
 __device__ inline uint32_t lds_load_relaxed(uint32_t* p) {
   return __builtin_amdgcn_readfirstlane(__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
 }
@@ -1193,7 +1204,7 @@ void transport_kernel(RunParams P) {
     for (;;) {
       const bool cold = L.mode != kInVoxel && L.mode != kDone;
       if (!__any(cold)) break;
-      if (cold) cold_step<NP, kSingle, kAcq, kTau>(P, T, L, c);
+      if (cold) cold_step<NP, kSingle, kAcq, kTau>(cold_params(), T, L, c);
     }
   } else {
     // The block works in rounds.  In each round every wave
@@ -1226,7 +1237,7 @@ void transport_kernel(RunParams P) {
       for (int pass = 0; pass < P.cold_passes; ++pass) {
         const bool cold = L.mode != kInVoxel && L.mode != kDone;
         if (!__any(cold)) break;
-        if (cold) cold_step<NP, kSingle, kAcq, kTau>(P, T, L, c);
+        if (cold) cold_step<NP, kSingle, kAcq, kTau>(cold_params(), T, L, c);
       }
       if (lane == 0) atomicAdd(&s_cold_done[rb], 1u);
       bool ready = false;
@@ -1251,7 +1262,7 @@ void transport_kernel(RunParams P) {
         // without the interaction code (measured), and its steps are cheap
         // enough that the waiting lanes cost less.
         if (!kTau && __popcll(__ballot(L.mode == kInteract)) >= P.interact_batch) {
-          if (L.mode == kInteract) resolve_interaction<NP, kTau>(P, T, L, c);
+          if (L.mode == kInteract) resolve_interaction<NP, kTau>(cold_params(), T, L, c);
         }
       }
       if constexpr (kTau) {
