@@ -94,6 +94,22 @@ struct DevSolid {
 
 static_assert(sizeof(DevSolid) % 16 == 0, "solids are staged in LDS after 16-byte table entries");
 
+// One dosel's tallies in one aligned 32-byte record (GGEMSDoseRecording's
+// edep, edep_squared and hit buffers, include/GGEMS/navigators/
+// GGEMSDoseRecording.hh:44-52).  The three atomics of a deposit then touch
+// one sector, so the dosel's line is fetched and written back once instead of
+// three times; ggh_get_dose_tallies splits the records into the three arrays.
+// edep^2 and hit are tallied whether or not they were requested: they ride in
+// the same sector, and the readback refuses the ones not requested.
+struct alignas(32) DoseTally {
+  double edep;
+  double edep2;
+  int32_t hit;
+  int32_t pad_[3];
+};
+
+static_assert(sizeof(DoseTally) == 32, "one dosel record per 32-byte sector");
+
 struct RunParams {
   const DevSolid* solids;
   int32_t n_solids;
@@ -127,9 +143,7 @@ struct RunParams {
   int32_t dose_solid;
   f3 dose_lo, dose_hi, inv_dosel;
   int32_t nd[3];
-  double* edep;
-  double* edep2;
-  int32_t* hit;
+  DoseTally* tally;
   int32_t* ptrack;
   // particle slots
   uint32_t* rng;              // SoA [5][rng_stride]
@@ -439,9 +453,10 @@ __device__ inline int dosel_index(const RunParams& P, f3 p) {
 __device__ inline void dose_record(const RunParams& P, float edep, f3 p) {
   int id = dosel_index(P, p);
   if (id < 0) return;
-  if (P.hit) atomicAdd(&P.hit[id], 1);
-  unsafeAtomicAdd(&P.edep[id], (double)edep);
-  if (P.edep2) unsafeAtomicAdd(&P.edep2[id], (double)edep * (double)edep);
+  DoseTally* t = P.tally + id;
+  atomicAdd(&t->hit, 1);
+  unsafeAtomicAdd(&t->edep, (double)edep);
+  unsafeAtomicAdd(&t->edep2, (double)edep * (double)edep);
 }
 
 // ---------------------------------------------------------------------------
@@ -1351,9 +1366,7 @@ namespace {
 constexpr int kDoseBlock = 256;
 
 struct DoseParams {
-  const double* edep;
-  const double* edep2;
-  const int32_t* hit;
+  const DoseTally* tally;
   const uint8_t* labels;
   const float* density;     // by global material
   int32_t mat_offset;
@@ -1368,6 +1381,18 @@ struct DoseParams {
   float* dose;
   float* unc;
 };
+
+// field 0, 1 or 2 (edep, edep^2, hit) of every record into a plain array
+__global__ __launch_bounds__(kDoseBlock) void tally_field_kernel(const DoseTally* t, int64_t n, int field, void* out) {
+  for (int64_t i = blockIdx.x * (int64_t)kDoseBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kDoseBlock) {
+    if (field == 0)
+      static_cast<double*>(out)[i] = t[i].edep;
+    else if (field == 1)
+      static_cast<double*>(out)[i] = t[i].edep2;
+    else
+      static_cast<int32_t*>(out)[i] = t[i].hit;
+  }
+}
 
 __global__ __launch_bounds__(kDoseBlock) void dose_kernel(DoseParams D) {
   const int64_t total = (int64_t)D.nd[0] * D.nd[1] * D.nd[2];
@@ -1391,13 +1416,14 @@ __global__ __launch_bounds__(kDoseBlock) void dose_kernel(DoseParams D) {
       mat = D.labels[vx + vy * D.nv[0] + vz * D.nv[0] * D.nv[1]];
     }
     float density = D.water_reference ? kWater : D.density[D.mat_offset + mat];
-    double e = D.edep[gid];
+    const DoseTally& t = D.tally[gid];
+    double e = t.edep;
     D.dose[gid] = density < D.min_density ? 0.0f : (float)((double)D.scale * e / (double)density / (double)vol / (double)kGy);
     if (D.unc) {
-      int h = D.hit[gid];
+      int h = t.hit;
       if (h > 1 && e != 0.0) {
         double s2 = e * e;
-        D.unc[gid] = (float)sqrt(((double)h * D.edep2[gid] - s2) / ((double)(h - 1) * s2));
+        D.unc[gid] = (float)sqrt(((double)h * t.edep2 - s2) / ((double)(h - 1) * s2));
       } else {
         D.unc[gid] = 1.0f;
       }
@@ -1746,8 +1772,8 @@ struct ggh_context {
   bool solids_dirty = true;
   // dosimetry
   ggh_dosimetry_desc dose_desc{};
-  DevBuf<double> edep, edep2;
-  DevBuf<int32_t> hit, ptrack;
+  DevBuf<DoseTally> tally;
+  DevBuf<int32_t> ptrack;
   // source
   Mat34 src_m{};
   float aperture = 0.0f;
@@ -1896,9 +1922,7 @@ RunParams make_params(ggh_context* c) {
     // inv_size_of_dosels_ = 1.0f / size (src/navigators/GGEMSDosimetryCalculator.cc:466-469)
     P.inv_dosel = mk(1.0f / d.dosel_size[0], 1.0f / d.dosel_size[1], 1.0f / d.dosel_size[2]);
     for (int i = 0; i < 3; ++i) P.nd[i] = d.n_dosels[i];
-    P.edep = c->edep.p;
-    P.edep2 = c->edep2.p;
-    P.hit = c->hit.p;
+    P.tally = c->tally.p;
     P.ptrack = c->ptrack.p;
   }
   P.rng = c->rng.p;
@@ -2156,7 +2180,7 @@ int ggh_context_destroy(ggh_context* c) {
   for (auto& b : c->mesh_tris) b.release();
   c->solids_dev.release();
   drop_acquisition(c);
-  c->edep.release(); c->edep2.release(); c->hit.release(); c->ptrack.release();
+  c->tally.release(); c->ptrack.release();
   c->spectrum.release(); c->cdf.release(); c->rng.release(); c->queue.release(); c->stats.release();
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -2537,12 +2561,8 @@ int ggh_set_dosimetry(ggh_context* c, const ggh_dosimetry_desc* d) {
     return fail(-2, "invalid dosel grid");
   drop_graph(c);  // the captured acquisition holds the tally pointers reallocated below
   c->dose_desc = *d;
-  if ((rc = dev_alloc(c->edep, n))) return rc;
-  c->edep2.release();
-  c->hit.release();
+  if ((rc = dev_alloc(c->tally, n))) return rc;
   c->ptrack.release();
-  if (d->edep_squared && (rc = dev_alloc(c->edep2, n))) return rc;
-  if (d->hit && (rc = dev_alloc(c->hit, n))) return rc;
   if (d->photon_tracking && (rc = dev_alloc(c->ptrack, n))) return rc;
   c->have_dose = true;
   return ggh_reset_tallies(c);
@@ -2599,9 +2619,7 @@ int ggh_reset_tallies(ggh_context* c) {
     if (c->hist[i].p) HIP_TRY(hipMemsetAsync(c->hist[i].p, 0, c->hist[i].n * sizeof(int32_t), c->stream));
     if (c->scat[i].p) HIP_TRY(hipMemsetAsync(c->scat[i].p, 0, c->scat[i].n * sizeof(int32_t), c->stream));
   }
-  if (c->edep.p) HIP_TRY(hipMemsetAsync(c->edep.p, 0, c->edep.n * sizeof(double), c->stream));
-  if (c->edep2.p) HIP_TRY(hipMemsetAsync(c->edep2.p, 0, c->edep2.n * sizeof(double), c->stream));
-  if (c->hit.p) HIP_TRY(hipMemsetAsync(c->hit.p, 0, c->hit.n * sizeof(int32_t), c->stream));
+  if (c->tally.p) HIP_TRY(hipMemsetAsync(c->tally.p, 0, c->tally.n * sizeof(DoseTally), c->stream));
   if (c->ptrack.p) HIP_TRY(hipMemsetAsync(c->ptrack.p, 0, c->ptrack.n * sizeof(int32_t), c->stream));
   for (auto* v : {&c->acq_hist, &c->acq_scat})
     for (auto& b : *v)
@@ -3036,20 +3054,31 @@ int ggh_get_dose_tallies(ggh_context* c, double* edep, double* edep2, int32_t* h
   int rc = check_ctx(c);
   if (rc) return rc;
   if (!c->have_dose) return fail(-2, "dosimetry not set");
-  const size_t n = c->edep.n;
-  if (edep) HIP_TRY(hipMemcpyAsync(edep, c->edep.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  if (edep2) {
-    if (!c->edep2.p) return fail(-2, "edep squared not stored");
-    HIP_TRY(hipMemcpyAsync(edep2, c->edep2.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (edep2 && !c->dose_desc.edep_squared) return fail(-2, "edep squared not stored");
+  if (hit && !c->dose_desc.hit) return fail(-2, "hits not stored");
+  if (ptrack && !c->ptrack.p) return fail(-2, "photon tracking not stored");
+  const size_t n = c->tally.n;
+  if (edep || edep2 || hit) {
+    // one field of the records at a time through a device staging array;
+    // the stream orders each split after the previous copy out of it
+    DevBuf<double> stage;
+    if ((rc = dev_alloc(stage, n))) return rc;
+    void* out[3] = {edep, edep2, hit};
+    const size_t bytes[3] = {sizeof(double), sizeof(double), sizeof(int32_t)};
+    const unsigned grid = (unsigned)std::min<size_t>((n + kDoseBlock - 1) / kDoseBlock, 65536);
+    hipError_t e = hipSuccess;
+    for (int f = 0; f < 3 && e == hipSuccess; ++f) {
+      if (!out[f]) continue;
+      hipLaunchKernelGGL(tally_field_kernel, dim3(grid), dim3(kDoseBlock), 0, c->stream, c->tally.p, (int64_t)n, f,
+                         (void*)stage.p);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipMemcpyAsync(out[f], stage.p, n * bytes[f], hipMemcpyDeviceToHost, c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    stage.release();
+    if (e != hipSuccess) return fail(-100 - (int)e, std::string("dose tallies: ") + hipGetErrorString(e));
   }
-  if (hit) {
-    if (!c->hit.p) return fail(-2, "hits not stored");
-    HIP_TRY(hipMemcpyAsync(hit, c->hit.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-  }
-  if (ptrack) {
-    if (!c->ptrack.p) return fail(-2, "photon tracking not stored");
-    HIP_TRY(hipMemcpyAsync(ptrack, c->ptrack.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-  }
+  if (ptrack) HIP_TRY(hipMemcpyAsync(ptrack, c->ptrack.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -3058,8 +3087,9 @@ int ggh_compute_dose(ggh_context* c, float scale, int32_t water_ref, float min_d
   int rc = check_ctx(c);
   if (rc) return rc;
   if (!c->have_dose) return fail(-2, "dosimetry not set");
-  if (unc && (!c->hit.p || !c->edep2.p)) return fail(-2, "uncertainty needs hit and edep squared tallies");
-  const size_t n = c->edep.n;
+  if (unc && (!c->dose_desc.hit || !c->dose_desc.edep_squared))
+    return fail(-2, "uncertainty needs hit and edep squared tallies");
+  const size_t n = c->tally.n;
   const DevSolid& s = c->solids[c->dose_desc.solid];
   DevBuf<float> d_dose, d_unc;
   if ((rc = dev_alloc(d_dose, n))) return rc;
@@ -3069,9 +3099,7 @@ int ggh_compute_dose(ggh_context* c, float scale, int32_t water_ref, float min_d
   }
   DoseParams D;
   std::memset(&D, 0, sizeof D);
-  D.edep = c->edep.p;
-  D.edep2 = c->edep2.p;
-  D.hit = c->hit.p;
+  D.tally = c->tally.p;
   D.labels = s.type == kVoxel ? s.vox.labels : nullptr;
   D.density = c->density.p;
   D.mat_offset = s.mat_offset;
