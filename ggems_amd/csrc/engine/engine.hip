@@ -1787,7 +1787,7 @@ struct ggh_context {
   DevBuf<unsigned long long> stats;
   // launch shape
   int round_steps = 20;
-  int cold_passes = 2;
+  int cold_passes = 0;  // 0: 1 with tau sampling, 2 with the reference sampling (measured best)
   int interact_batch = 16;
   int blocks_per_cu = 0;  // 0: as many as fit
   bool force_general = false;
@@ -1930,7 +1930,7 @@ RunParams make_params(ggh_context* c) {
   P.queue = c->queue.p;
   P.stats = c->stats.p;
   P.round_steps = c->round_steps;
-  P.cold_passes = c->cold_passes;
+  P.cold_passes = c->cold_passes ? c->cold_passes : (c->tau_mode ? 1 : 2);
   P.interact_batch = c->interact_batch;
   P.xchg_offset = 0;
   for (const DevSolid& s : c->solids)
@@ -2645,7 +2645,7 @@ int ggh_set_launch(ggh_context* c, const ggh_launch_desc* d) {
       d->interact_batch < 0 || d->interact_batch > 64 || d->blocks_per_cu < 0 || d->blocks_per_cu > 64)
     return fail(-2, "invalid launch parameters");
   c->round_steps = d->round_steps ? d->round_steps : 20;
-  c->cold_passes = d->cold_passes ? d->cold_passes : 2;
+  c->cold_passes = d->cold_passes;
   c->interact_batch = d->interact_batch ? d->interact_batch : 16;
   c->blocks_per_cu = d->blocks_per_cu;
   c->force_general = d->general_kernel != 0;

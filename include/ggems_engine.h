@@ -168,7 +168,8 @@ typedef struct {
   int32_t round_steps;     /* voxel steps a wave makes at least between two regroupings of its
                               block's photons (1..1024, default 20) */
   int32_t cold_passes;     /* navigation passes per round for the photons that are not inside a
-                              phantom (1..16, default 2) */
+                              phantom (1..16; default 2 with the reference sampling, 1 with tau
+                              sampling) */
   int32_t interact_batch;  /* inside the voxel loop, resolve pending Compton/Rayleigh/photoelectric
                               interactions once this many lanes wait for one (default 16; 64 leaves
                               them to the navigation passes) */
