@@ -1273,21 +1273,25 @@ void transport_kernel(RunParams P) {
       }
       if (lane == 0) atomicAdd(&s_cold_done[rb], 1u);
       bool ready = false;
-      for (int k = 0;; ++k) {
+      for (int k = 0;; k += kTau ? 2 : 1) {  // k counts voxel steps
         const uint64_t act = __ballot(L.mode == kInVoxel);
         if (!ready && (act == 0 || (k >= P.round_steps && lds_load_relaxed(&s_cold_done[rb]) == (uint32_t)kWaves))) {
           ready = true;
           if (lane == 0) atomicAdd(&s_ready[rb], 1u);
         }
         if (act == 0 || (ready && lds_load_relaxed(&s_ready[rb]) == (uint32_t)kWaves)) break;
-        bool stepped = false;
+        bool stepped = false, stepped2 = false;
         // the phantom's geometry: kernel arguments for the only voxel
         // solid, else the lane's solid in the (LDS-staged) solid table
         if (L.mode == kInVoxel) {
           const VoxGeom& G = kSingle ? P.vox0 : T.solids[L.solid].vox;
           stepped = kTau ? hot_step_tau<kTrack>(P, T, G, L) : hot_step<NP, kTrack>(P, T, G, L);
+          // tau sampling makes two steps per pass: its step is short, so the
+          // pass's ballots and round checks weigh more (measured faster; three
+          // gained little more, four was slower)
+          if (kTau && L.mode == kInVoxel) stepped2 = hot_step_tau<kTrack>(P, T, G, L);
         }
-        steps += __popcll(__ballot(stepped));
+        steps += __popcll(__ballot(stepped)) + (kTau ? __popcll(__ballot(stepped2)) : 0);
         // pending interactions resolved in batches, so that scattered photons
         // keep stepping without waiting for the next round.  Tau sampling
         // leaves them to the navigation passes: its voxel loop runs faster
@@ -1818,7 +1822,7 @@ struct ggh_context {
   DevBuf<uint32_t> queue;
   DevBuf<unsigned long long> stats;
   // launch shape
-  int round_steps = 20;
+  int round_steps = 0;  // 0: 16 with tau sampling, 20 with the reference sampling (measured best)
   int cold_passes = 0;  // 0: 1 with tau sampling, 2 with the reference sampling (measured best)
   int interact_batch = 16;
   int blocks_per_cu = 0;  // 0: as many as fit
@@ -1963,7 +1967,7 @@ RunParams make_params(ggh_context* c) {
   P.rng_stride = GGH_MAX_SLOTS;
   P.queue = c->queue.p;
   P.stats = c->stats.p;
-  P.round_steps = c->round_steps;
+  P.round_steps = c->round_steps ? c->round_steps : (c->tau_mode ? 16 : 20);
   P.cold_passes = c->cold_passes ? c->cold_passes : (c->tau_mode ? 1 : 2);
   P.interact_batch = c->interact_batch;
   P.xchg_offset = 0;
@@ -2700,7 +2704,7 @@ int ggh_set_launch(ggh_context* c, const ggh_launch_desc* d) {
   if (!d || d->round_steps < 0 || d->round_steps > 1024 || d->cold_passes < 0 || d->cold_passes > 16 ||
       d->interact_batch < 0 || d->interact_batch > 64 || d->blocks_per_cu < 0 || d->blocks_per_cu > 64)
     return fail(-2, "invalid launch parameters");
-  c->round_steps = d->round_steps ? d->round_steps : 20;
+  c->round_steps = d->round_steps;
   c->cold_passes = d->cold_passes;
   c->interact_batch = d->interact_batch ? d->interact_batch : 16;
   c->blocks_per_cu = d->blocks_per_cu;

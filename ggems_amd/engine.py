@@ -401,8 +401,8 @@ class Engine:
         return pos, dir, e
 
     def set_launch(self, round_steps=0, interact_batch=0, blocks_per_cu=0, general_kernel=False, cold_passes=0):
-        """launch shape of run(); zero keeps a default (round_steps 20, interact_batch 16, cold_passes 2
-        with the reference sampling and 1 with tau sampling)"""
+        """launch shape of run(); zero keeps a default (interact_batch 16; round_steps 20 and
+        cold_passes 2 with the reference sampling, 16 and 1 with tau sampling)"""
         d = LaunchDesc(int(round_steps), int(cold_passes), int(interact_batch), int(blocks_per_cu),
                        int(bool(general_kernel)))
         _check(library().ggh_set_launch(self.h, C.byref(d)))

@@ -166,7 +166,8 @@ int ggh_get_stats(ggh_context* ctx, ggh_stats* out);
  * Zero fields keep the defaults. */
 typedef struct {
   int32_t round_steps;     /* voxel steps a wave makes at least between two regroupings of its
-                              block's photons (1..1024, default 20) */
+                              block's photons (1..1024; default 20 with the reference sampling, 16
+                              with tau sampling) */
   int32_t cold_passes;     /* navigation passes per round for the photons that are not inside a
                               phantom (1..16; default 2 with the reference sampling, 1 with tau
                               sampling) */
