@@ -16,10 +16,11 @@
 // is the same kernel on the flipped, transposed weights.
 //
 // Weight gradient (M = Cout, N = 9 Cin, K = pixels): both operands have
-// pixels as their strided index, so a block stages 32 pixels of dY and of
-// the shifted input in LDS transposed (pixels contiguous) and the waves
-// read MFMA fragments from there; blocks split the pixels and add their
-// partial sums atomically in FP32.
+// pixels as their strided index.  A block stages 32 pixels of dY and of the
+// shifted input in LDS as they lie in memory (channels contiguous, 16-byte
+// stores) and the waves read MFMA fragments with gfx950's transposing LDS
+// read; blocks split the pixels and add their partial sums atomically in
+// FP32.
 
 #include <hip/hip_runtime.h>
 
@@ -225,6 +226,31 @@ __global__ __launch_bounds__(64 * kWaves) void conv_fwd_splitk_kernel(const __bf
 constexpr int kWgPix = 32;   // pixels per K step
 constexpr int kWgN = 64;     // (tap, ci) columns per block
 constexpr int kWgMaxM = 128; // output channels per block (all of them)
+// staged rows: [pixel][channel], padded by 32 bytes so that the 16 rows one
+// transposing read touches per half-wave start 8 banks apart
+constexpr int kWgDyRow = kWgMaxM + 16;
+constexpr int kWgXRow = kWgN + 16;
+
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __attribute__((address_space(3))) s16x4 lds_s16x4;
+
+// ds_read_b64_tr_b16 (gfx950): within each group of 16 lanes, element e of
+// lane i's result is 16-bit element i % 4 of the 8 bytes that lane
+// 4 e + i / 4 of the group addressed (checked on the hardware).  Lane (g, j)
+// of an MFMA fragment (j = lane % 16, g = lane / 16) therefore addresses
+// row 4 g + j / 4 (+ 16 for the second half), column col0 + 4 (j % 4), and
+// receives column col0 + j of rows 4 g .. 4 g + 3 and 16 + 4 g .. 16 + 4 g
+// + 3: the fragment's k = 8 g + t is staged pixel 16 (t / 4) + 4 g + t % 4.
+// Both operands use this order, so the sum over pixels is unchanged.
+__device__ inline bf16x8 frag_tr(const __bf16* tile, int row_len, int col0, int g, int j) {
+  const __bf16* p0 = tile + (4 * g + (j >> 2)) * row_len + col0 + 4 * (j & 3);
+  const __bf16* p1 = p0 + 16 * row_len;
+  const s16x4 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s16x4*)p0);
+  const s16x4 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s16x4*)p1);
+  const s16x8 v = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+  return __builtin_bit_cast(bf16x8, v);
+}
 
 __global__ __launch_bounds__(64 * kWaves) void conv_wgrad_kernel(const __bf16* __restrict__ x,
                                                                  const __bf16* __restrict__ dy,
@@ -232,8 +258,8 @@ __global__ __launch_bounds__(64 * kWaves) void conv_wgrad_kernel(const __bf16* _
                                                                  int Cin, int Cout, int pix_per_split) {
   // double-buffered: step k + 1's global loads are in flight while step k's
   // MFMAs run, and land in the other buffer
-  __shared__ __bf16 s_dy[2][kWgMaxM][kWgPix + 8];  // [buf][co][pixel], padded rows
-  __shared__ __bf16 s_x[2][kWgN][kWgPix + 8];      // [buf][n][pixel]
+  __shared__ __attribute__((aligned(16))) __bf16 s_dy[2][kWgPix * kWgDyRow];  // [buf][pixel][co]
+  __shared__ __attribute__((aligned(16))) __bf16 s_x[2][kWgPix * kWgXRow];    // [buf][pixel][n]
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int K9 = 9 * Cin;
   const int n0 = blockIdx.x * kWgN;
@@ -270,17 +296,17 @@ __global__ __launch_bounds__(64 * kWaves) void conv_wgrad_kernel(const __bf16* _
       }
     }
   };
-  auto store = [&](int buf) {  // transposed: pixels contiguous
+  auto store = [&](int buf) {  // as loaded: channels contiguous, one 16-byte store per piece
     for (int q = 0; q < kDyPieces; ++q) {
       const int t = threadIdx.x + q * 64 * kWaves;
       if (t < n_dy) {
         const int p = t / (Cout / 8), c8 = (t - p * (Cout / 8)) * 8;
-        for (int e = 0; e < 8; ++e) s_dy[buf][c8 + e][p] = vdy[q][e];
+        *reinterpret_cast<bf16x8*>(&s_dy[buf][p * kWgDyRow + c8]) = vdy[q];
       }
     }
     const int t = threadIdx.x;
     const int p = t / (kWgN / 8), n8 = (t - p * (kWgN / 8)) * 8;
-    for (int e = 0; e < 8; ++e) s_x[buf][n8 + e][p] = vx[e];
+    *reinterpret_cast<bf16x8*>(&s_x[buf][p * kWgXRow + n8]) = vx;
   };
   // wave w owns column tile w (16 of the block's 64 columns) and every row tile
   f32x4 acc[kWgMaxM / 16];
@@ -294,14 +320,11 @@ __global__ __launch_bounds__(64 * kWaves) void conv_wgrad_kernel(const __bf16* _
   for (int pc = pbeg; pc < pend; pc += kWgPix) {
     const bool more = pc + kWgPix < pend;
     if (more) load(pc + kWgPix);
-    // B fragment: column (n) r of the wave's tile, pixels 8 g .. 8 g + 7
-    bf16x8 b;
-    for (int e = 0; e < 8; ++e) b[e] = s_x[buf][16 * wave + r][8 * g + e];
+    const bf16x8 b = frag_tr(s_x[buf], kWgXRow, 16 * wave, g, r);
 #pragma unroll
     for (int i = 0; i < kWgMaxM / 16; ++i) {
       if (i >= m_tiles) break;
-      bf16x8 a;
-      for (int e = 0; e < 8; ++e) a[e] = s_dy[buf][16 * i + r][8 * g + e];
+      const bf16x8 a = frag_tr(s_dy[buf], kWgDyRow, 16 * i, g, r);
       acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i], 0, 0, 0);
     }
     if (more) store(buf ^ 1);  // the other buffer: nobody reads it during this step
