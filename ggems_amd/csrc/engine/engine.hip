@@ -820,20 +820,29 @@ __device__ void track_mesh(const RunParams& P, const Tabs& T, const DevSolid& s,
 // ---------------------------------------------------------------------------
 // source (src/kernels/GetPrimariesGGEMSXRaySource.cl:52-143)
 // ---------------------------------------------------------------------------
-template <bool kAcq>
+template <bool kAcq, bool kTau>
 __device__ void sample_primary(const RunParams& P, const Tabs& T, const Mat34& src_m, Rng& r, f3& pos, f3& dir,
                                float& energy, int& eid) {
   double phi = r.uniform();
   double theta = r.uniform();
   phi *= (double)kTwoPi;
-  // the acquisition variant reads 1 - cos(aperture) from the host; the
-  // plain-run variant recomputes it, which measured faster there (register
-  // allocation).  The host's and the device's FP64 cos agree on every
-  // aperture the tests use.
-  const double new_aperture = kAcq ? P.new_aperture : 1.0 - cos((double)P.aperture);
+  // The acquisition and tau-sampling variants read 1 - cos(aperture) from
+  // the host and take sin and cos of phi from one sincos (the same values:
+  // one range reduction for both); the plain reference-sampling variant
+  // recomputes the first and calls sin and cos, which measured faster there
+  // (register allocation).  The host's and the device's FP64 cos agree on
+  // every aperture the tests use.
+  const double new_aperture = (kAcq || kTau) ? P.new_aperture : 1.0 - cos((double)P.aperture);
   theta = acos(1.0 - new_aperture * theta);
   double st = sin(theta);
-  f3 rotation = mk((float)(cos(phi) * st), (float)(sin(phi) * st), (float)cos(theta));
+  double sp, cp;
+  if (kTau) {
+    sincos(phi, &sp, &cp);
+  } else {
+    sp = sin(phi);
+    cp = cos(phi);
+  }
+  f3 rotation = mk((float)(cp * st), (float)(sp * st), (float)cos(theta));
   f3 gp = l2g_pos(src_m, mk(0.0f, 0.0f, 0.0f));
   f3 d = normalize(mk(0.0f - gp.x, 0.0f - gp.y, 0.0f - gp.z));
   d = normalize(rotate_unit_z(rotation, d));
@@ -1042,7 +1051,7 @@ __device__ inline void cold_step(const RunParams& P, const Tabs& T, Lane& L, Cou
       L.mode = kNeedSlot;  // empty slot: take another one on the next pass
     } else {
       --L.todo;
-      sample_primary<kAcq>(P, T, kAcq ? P.acq_src[L.proj] : P.src_m, L.r, L.p, L.ray.d, L.e, L.eid);
+      sample_primary<kAcq, kTau>(P, T, kAcq ? P.acq_src[L.proj] : P.src_m, L.r, L.p, L.ray.d, L.e, L.eid);
       c.add(kCntPrimaries);
       L.scatter = 0;
       L.hop = 0;
@@ -1698,7 +1707,7 @@ __global__ void primaries_kernel(RunParams P, int n, float* pos, float* dir, flo
   float energy;
   int eid;
   const Tabs T = {P.tab_sig, P.tab_rcp, P.ebins, P.solids, P.spectrum, P.cdf};
-  sample_primary<false>(P, T, P.src_m, r, gp, gd, energy, eid);
+  sample_primary<false, false>(P, T, P.src_m, r, gp, gd, energy, eid);
   pos[3 * s] = gp.x; pos[3 * s + 1] = gp.y; pos[3 * s + 2] = gp.z;
   dir[3 * s] = gd.x; dir[3 * s + 1] = gd.y; dir[3 * s + 2] = gd.z;
   e[s] = energy;
