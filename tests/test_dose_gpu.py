@@ -21,6 +21,7 @@ import pytest
 
 import ggems_oracle as O
 from ggems_amd import authoring
+from ggems_amd.engine import library
 from scene import REPO, Scene, cfg1_scene, voxel_ct_scene
 
 pytestmark = pytest.mark.gpu
@@ -130,4 +131,36 @@ def test_full_grid_dose_matches_oracle():
     eng = body.make_engine()
     body.run_engine(10_000_000, 4711, eng)
     _check_against_oracle(body, eng, [(1.0, False, 0.0), (1.0, True, F(0.5 * G_CM3))])
+    eng.close()
+
+
+def test_edep_only_dosimetry_tallies_edep_and_refuses_the_rest():
+    """Dosimetry with edep squared and hits off (GGEMSDosimetryCalculator's
+    defaults for a script that activates neither) tallies the same edep as
+    a run with both on, and the readback refuses the inactive tallies and
+    the uncertainty that needs them."""
+    sc = cfg1_scene(n_side=60, detector=64)
+    full = sc.run_engine(200_000, 91)
+    eng = sc.make_engine()
+    k = sc.engine_solids["voxel"][0]
+    d = sc.dose_grid(sc.navigators[0][2])
+    pay = sc.navigators[0][2]
+    eng.set_dosimetry(k, d["size"], d["n"], pay["bmin"], pay["bmax"], edep_squared=False, hit=False)
+    part = sc.run_engine(200_000, 91, eng)
+    assert part["edep2"] is None and part["hit"] is None
+    assert part["stats"] == full["stats"]
+    # the same deposits, summed by FP64 atomics in another order
+    np.testing.assert_allclose(part["edep"], full["edep"], rtol=1e-12, atol=1e-18)
+    assert part["edep"].sum() > 0
+    lib = library()
+    nx, ny, nz = eng.n_dosels
+    buf = np.zeros(nx * ny * nz, np.float64)
+    ibuf = np.zeros(nx * ny * nz, np.int32)
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    assert lib.ggh_get_dose_tallies(eng.h, None, p(buf), None, None) == -2
+    assert lib.ggh_get_dose_tallies(eng.h, None, None, p(ibuf), None) == -2
+    fbuf = np.zeros(nx * ny * nz, np.float32)
+    assert lib.ggh_compute_dose(eng.h, C.c_float(1.0), 0, C.c_float(0.0), p(fbuf), p(fbuf.copy())) == -2
+    assert lib.ggh_compute_dose(eng.h, C.c_float(1.0), 0, C.c_float(0.0), p(fbuf), None) == 0
+    assert fbuf.sum() > 0
     eng.close()
