@@ -288,7 +288,7 @@ def main():
     ap.add_argument("--seed", type=int, default=777)
     ap.add_argument("--workdir", default=os.path.join(REPO, "ggems_amd", "data", "_gen", "voxel_ct"))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    # the GPU box gives each GPU 16 host CPUs (OMP_NUM_THREADS there); elsewhere every core
+    # the host's share of CPUs for one GPU where OMP_NUM_THREADS states it; otherwise every core
     ap.add_argument("--cpu-threads", type=int,
                     default=int(os.environ.get("OMP_NUM_THREADS") or 0) or (os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
