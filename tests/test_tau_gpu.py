@@ -72,6 +72,38 @@ def test_tau_two_phantoms_and_photon_tracking_match_oracle():
     eng.close()
 
 
+def test_tau_ragged_phantom_matches_oracle():
+    """37 x 50 x 23 voxels of 1.1 x 0.7 x 1.6 mm, rotated: the tau step reads
+    the labels from 8 x 4 x 4-voxel bricks, and every axis ends in a partly
+    filled brick here (5, 13 and 6 bricks), so a wrong brick stride or
+    padding changes histories against the oracle, which reads the labels
+    row-major.  Three materials in seeded blobs make every label matter."""
+    g = np.random.default_rng(11)
+    lab = np.zeros((23, 50, 37), np.uint8)
+    zz, yy, xx = np.meshgrid(np.arange(23), np.arange(50), np.arange(37), indexing="ij")
+    for _ in range(12):
+        c = g.uniform([0, 0, 0], [23, 50, 37])
+        r = g.uniform(3.0, 9.0)
+        lab[(zz - c[0]) ** 2 + (yy - c[1]) ** 2 + (xx - c[2]) ** 2 < r * r] = g.integers(1, 3)
+    sc = Scene()
+    sc.add_phantom(lab, ["Water", "RibBone", "Lung"], (1.1, 0.7, 1.6), rotation=(10.0, 0.0, 25.0),
+                   dosimetry={"photon_tracking": False})
+    sc.add_ct_flat("GOS", (1, 1), (96, 96, 1), (1.0, 1.0, 1.0), 300.0, 500.0, 10e-3)
+    sc.set_source((-300.0, 0.0, 0.0), 4.0, [0.05], mono=True)
+    _tau(sc.build())
+    n = 200_000
+    ref = sc.run_oracle(n, 4242, threads=0)
+    eng = sc.make_engine()
+    got = sc.run_engine(n, 4242, eng)
+    assert eng.last_grid()[2] == 0  # the specialised kernel: the bricks of the only voxel solid
+    assert ref["stats"]["voxel_steps"] > 10 * n
+    parity.counters("tau ragged phantom", ref["stats"], got["stats"], ("voxel_steps", "interactions", "hops"), 1e-3)
+    parity.l1("tau ragged phantom histogram", ref["hist"][0], got["hist"][0], 1e-3, 10)
+    parity.l1("tau ragged phantom hit", ref["hit"], got["hit"], 2e-3, 10)
+    parity.total("tau ragged phantom edep", ref["edep"], got["edep"], 1e-3)
+    eng.close()
+
+
 def test_tau_launch_shape_does_not_change_histories():
     sc = _tau(cfg1_scene(n_side=60, detector=64))
     eng = sc.make_engine()
