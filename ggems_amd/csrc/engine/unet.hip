@@ -400,6 +400,139 @@ __global__ __launch_bounds__(kMaskBlock) void relu_bias_backward_kernel(const vo
   for (int i = threadIdx.x; i < C; i += kMaskBlock) unsafeAtomicAdd(&db[i], s_db[i]);
 }
 
+// The U-Net's 2 x 2 max pooling and its nearest-neighbour upsampling with
+// the skip concatenation, NHWC BF16, one thread per 8 channels of a pixel:
+// bandwidth kernels in place of the strided copies, reductions and masks
+// that the same steps take as tensor operations.  The pooling keeps, per
+// channel, which input of the window (row-major: 0 top left .. 3 bottom
+// right) held the first maximum, and its backward routes the gradient
+// there, as max_pool2d does.
+constexpr int kGlueBlock = 256;
+
+unsigned glue_grid(long long pieces) {
+  const long long b = (pieces + kGlueBlock - 1) / kGlueBlock;
+  return (unsigned)(b < 4096 ? (b > 0 ? b : 1) : 4096);
+}
+
+__global__ __launch_bounds__(kGlueBlock) void pool2x2_kernel(const __bf16* __restrict__ x, __bf16* __restrict__ y,
+                                                             uint8_t* __restrict__ arg, int n, int h, int w, int c) {
+  const int ho = h / 2, wo = w / 2, g = c / 8;
+  const long long total = (long long)n * ho * wo * g;
+  for (long long q = blockIdx.x * (long long)kGlueBlock + threadIdx.x; q < total;
+       q += (long long)gridDim.x * kGlueBlock) {
+    const int c8 = (int)(q % g) * 8;
+    long long p = q / g;  // output pixel
+    const int xo = (int)(p % wo);
+    p /= wo;
+    const int yo = (int)(p % ho);
+    const long long img = p / ho;
+    const __bf16* base = x + ((img * h + 2 * yo) * (long long)w + 2 * xo) * c + c8;
+    bf16x8 v[4];
+    v[0] = *reinterpret_cast<const bf16x8*>(base);
+    v[1] = *reinterpret_cast<const bf16x8*>(base + c);
+    v[2] = *reinterpret_cast<const bf16x8*>(base + (long long)w * c);
+    v[3] = *reinterpret_cast<const bf16x8*>(base + (long long)w * c + c);
+    bf16x8 m;
+    uint64_t a = 0;
+    for (int e = 0; e < 8; ++e) {
+      float best = (float)v[0][e];
+      int bi = 0;
+      for (int k = 1; k < 4; ++k) {
+        const float t = (float)v[k][e];
+        if (t > best) {
+          best = t;
+          bi = k;
+        }
+      }
+      m[e] = v[bi][e];
+      a |= (uint64_t)bi << (8 * e);
+    }
+    *reinterpret_cast<bf16x8*>(y + q * 8) = m;  // output pixel p, channels c8.. : element p c + c8 = 8 q
+    *reinterpret_cast<uint64_t*>(arg + q * 8) = a;
+  }
+}
+
+__global__ __launch_bounds__(kGlueBlock) void pool2x2_backward_kernel(const __bf16* __restrict__ dy,
+                                                                      const uint8_t* __restrict__ arg,
+                                                                      __bf16* __restrict__ dx, int n, int h, int w,
+                                                                      int c) {
+  const int ho = h / 2, wo = w / 2, g = c / 8;
+  const long long total = (long long)n * ho * wo * g;
+  for (long long q = blockIdx.x * (long long)kGlueBlock + threadIdx.x; q < total;
+       q += (long long)gridDim.x * kGlueBlock) {
+    const int c8 = (int)(q % g) * 8;
+    long long p = q / g;
+    const int xo = (int)(p % wo);
+    p /= wo;
+    const int yo = (int)(p % ho);
+    const long long img = p / ho;
+    const bf16x8 d = *reinterpret_cast<const bf16x8*>(dy + q * 8);
+    const uint64_t a = *reinterpret_cast<const uint64_t*>(arg + q * 8);
+    __bf16* base = dx + ((img * h + 2 * yo) * (long long)w + 2 * xo) * c + c8;
+    __bf16* out[4] = {base, base + c, base + (long long)w * c, base + (long long)w * c + c};
+    for (int k = 0; k < 4; ++k) {
+      bf16x8 v;
+      for (int e = 0; e < 8; ++e) v[e] = ((a >> (8 * e)) & 0xffu) == (uint64_t)k ? d[e] : (__bf16)0.0f;
+      *reinterpret_cast<bf16x8*>(out[k]) = v;
+    }
+  }
+}
+
+// out [n][h][w][cz + cs] = (z upsampled 2x by repetition, s): the decoder's
+// input, channels of z first
+__global__ __launch_bounds__(kGlueBlock) void upcat_kernel(const __bf16* __restrict__ z, const __bf16* __restrict__ s,
+                                                           __bf16* __restrict__ out, int n, int h, int w, int cz,
+                                                           int cs) {
+  const int g = (cz + cs) / 8;
+  const long long total = (long long)n * h * w * g;
+  for (long long q = blockIdx.x * (long long)kGlueBlock + threadIdx.x; q < total;
+       q += (long long)gridDim.x * kGlueBlock) {
+    const int c8 = (int)(q % g) * 8;
+    long long p = q / g;
+    const int xx = (int)(p % w);
+    p /= w;
+    const int yy = (int)(p % h);
+    const long long img = p / h;
+    const bf16x8 v = c8 < cz
+        ? *reinterpret_cast<const bf16x8*>(z + ((img * (h / 2) + yy / 2) * (long long)(w / 2) + xx / 2) * cz + c8)
+        : *reinterpret_cast<const bf16x8*>(s + ((img * h + yy) * (long long)w + xx) * cs + (c8 - cz));
+    *reinterpret_cast<bf16x8*>(out + q * 8) = v;
+  }
+}
+
+// backward of upcat: dz = the sum of dout's z channels over each 2 x 2 block
+// (FP32, one rounding), ds = dout's s channels
+__global__ __launch_bounds__(kGlueBlock) void upcat_backward_kernel(const __bf16* __restrict__ dout,
+                                                                    __bf16* __restrict__ dz, __bf16* __restrict__ ds,
+                                                                    int n, int h, int w, int cz, int cs) {
+  const int ct = cz + cs, gz = cz / 8, gs = cs / 8;
+  const long long nz = (long long)n * (h / 2) * (w / 2) * gz, ns = (long long)n * h * w * gs;
+  for (long long q = blockIdx.x * (long long)kGlueBlock + threadIdx.x; q < nz + ns;
+       q += (long long)gridDim.x * kGlueBlock) {
+    if (q < nz) {
+      const int c8 = (int)(q % gz) * 8;
+      long long p = q / gz;
+      const int xo = (int)(p % (w / 2));
+      p /= (w / 2);
+      const int yo = (int)(p % (h / 2));
+      const long long img = p / (h / 2);
+      const __bf16* base = dout + ((img * h + 2 * yo) * (long long)w + 2 * xo) * ct + c8;
+      const bf16x8 v0 = *reinterpret_cast<const bf16x8*>(base);
+      const bf16x8 v1 = *reinterpret_cast<const bf16x8*>(base + ct);
+      const bf16x8 v2 = *reinterpret_cast<const bf16x8*>(base + (long long)w * ct);
+      const bf16x8 v3 = *reinterpret_cast<const bf16x8*>(base + (long long)w * ct + ct);
+      bf16x8 o;
+      for (int e = 0; e < 8; ++e) o[e] = (__bf16)((((float)v0[e] + (float)v1[e]) + (float)v2[e]) + (float)v3[e]);
+      *reinterpret_cast<bf16x8*>(dz + q * 8) = o;
+    } else {
+      const long long r = q - nz;
+      const int c8 = (int)(r % gs) * 8;
+      const long long p = r / gs;
+      *reinterpret_cast<bf16x8*>(ds + r * 8) = *reinterpret_cast<const bf16x8*>(dout + p * ct + cz + c8);
+    }
+  }
+}
+
 // The BF16 operands of a layer's forward and data-gradient convolutions from
 // its FP32 master weights, in one pass: w [cout][9][cin] rounded to nearest
 // even, and the data gradient's weights [cin][9][cout] with the taps
@@ -505,6 +638,59 @@ int ggh_conv3x3_weight_grad(const void* x, const void* dy, float* dw, int32_t n,
                      static_cast<const __bf16*>(x), static_cast<const __bf16*>(dy), dw, n, h, wd, cin, cout, per);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return ufail(-100 - (int)e, std::string("conv weight gradient: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int ggh_pool2x2_forward(const void* x, void* y, uint8_t* arg, int32_t n, int32_t h, int32_t wd, int32_t c,
+                        void* stream) {
+  if (!x || !y || !arg) return ufail(-1, "null pointer");
+  if (n <= 0 || h <= 0 || wd <= 0 || h % 2 || wd % 2 || c <= 0 || c % 8) return ufail(-2, "unsupported shape");
+  const long long pieces = (long long)n * (h / 2) * (wd / 2) * (c / 8);
+  hipLaunchKernelGGL(pool2x2_kernel, dim3(glue_grid(pieces)), dim3(kGlueBlock), 0, static_cast<hipStream_t>(stream),
+                     static_cast<const __bf16*>(x), static_cast<__bf16*>(y), arg, n, h, wd, c);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return ufail(-100 - (int)e, std::string("pooling: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int ggh_pool2x2_backward(const void* dy, const uint8_t* arg, void* dx, int32_t n, int32_t h, int32_t wd, int32_t c,
+                         void* stream) {
+  if (!dy || !arg || !dx) return ufail(-1, "null pointer");
+  if (n <= 0 || h <= 0 || wd <= 0 || h % 2 || wd % 2 || c <= 0 || c % 8) return ufail(-2, "unsupported shape");
+  const long long pieces = (long long)n * (h / 2) * (wd / 2) * (c / 8);
+  hipLaunchKernelGGL(pool2x2_backward_kernel, dim3(glue_grid(pieces)), dim3(kGlueBlock), 0,
+                     static_cast<hipStream_t>(stream), static_cast<const __bf16*>(dy), arg, static_cast<__bf16*>(dx), n,
+                     h, wd, c);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return ufail(-100 - (int)e, std::string("pooling backward: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int ggh_upcat_forward(const void* z, const void* s, void* out, int32_t n, int32_t h, int32_t wd, int32_t cz,
+                      int32_t cs, void* stream) {
+  if (!z || !s || !out) return ufail(-1, "null pointer");
+  if (n <= 0 || h <= 0 || wd <= 0 || h % 2 || wd % 2 || cz <= 0 || cz % 8 || cs <= 0 || cs % 8)
+    return ufail(-2, "unsupported shape");
+  const long long pieces = (long long)n * h * wd * ((cz + cs) / 8);
+  hipLaunchKernelGGL(upcat_kernel, dim3(glue_grid(pieces)), dim3(kGlueBlock), 0, static_cast<hipStream_t>(stream),
+                     static_cast<const __bf16*>(z), static_cast<const __bf16*>(s), static_cast<__bf16*>(out), n, h, wd,
+                     cz, cs);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return ufail(-100 - (int)e, std::string("upsampling: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int ggh_upcat_backward(const void* dout, void* dz, void* ds, int32_t n, int32_t h, int32_t wd, int32_t cz, int32_t cs,
+                       void* stream) {
+  if (!dout || !dz || !ds) return ufail(-1, "null pointer");
+  if (n <= 0 || h <= 0 || wd <= 0 || h % 2 || wd % 2 || cz <= 0 || cz % 8 || cs <= 0 || cs % 8)
+    return ufail(-2, "unsupported shape");
+  const long long pieces = (long long)n * (h / 2) * (wd / 2) * (cz / 8) + (long long)n * h * wd * (cs / 8);
+  hipLaunchKernelGGL(upcat_backward_kernel, dim3(glue_grid(pieces)), dim3(kGlueBlock), 0,
+                     static_cast<hipStream_t>(stream), static_cast<const __bf16*>(dout), static_cast<__bf16*>(dz),
+                     static_cast<__bf16*>(ds), n, h, wd, cz, cs);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return ufail(-100 - (int)e, std::string("upsampling backward: ") + hipGetErrorString(e));
   return 0;
 }
 

@@ -36,6 +36,10 @@ _SIGS = {
     "ggh_relu_bias_backward": ([C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64,
                                 C.c_int32, C.c_void_p], C.c_int),
     "ggh_conv3x3_weights_bf16": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p], C.c_int),
+    "ggh_pool2x2_forward": ([C.c_void_p, C.c_void_p, C.c_void_p] + [C.c_int32] * 4 + [C.c_void_p], C.c_int),
+    "ggh_pool2x2_backward": ([C.c_void_p, C.c_void_p, C.c_void_p] + [C.c_int32] * 4 + [C.c_void_p], C.c_int),
+    "ggh_upcat_forward": ([C.c_void_p, C.c_void_p, C.c_void_p] + [C.c_int32] * 5 + [C.c_void_p], C.c_int),
+    "ggh_upcat_backward": ([C.c_void_p, C.c_void_p, C.c_void_p] + [C.c_int32] * 5 + [C.c_void_p], C.c_int),
 }
 
 
@@ -160,14 +164,74 @@ class Conv3x3(torch.nn.Module):
         return _Conv3x3.apply(x, self.weight, self.bias, self.relu)
 
 
-def _pool(x):
-    n, h, w, c = x.shape
-    return x.view(n, h // 2, 2, w // 2, 2, c).amax(dim=(2, 4))
+def _bf16(t):
+    return t.to(torch.bfloat16).contiguous()
 
 
-def _up(x):
+def pool2x2(x):
+    """2 x 2 max pooling of NHWC BF16: (y [N,H/2,W/2,C], the window index of
+    each output's first maximum, uint8)"""
+    x = _bf16(x)
     n, h, w, c = x.shape
-    return x[:, :, None, :, None, :].expand(n, h, 2, w, 2, c).reshape(n, 2 * h, 2 * w, c)
+    y = torch.empty((n, h // 2, w // 2, c), dtype=torch.bfloat16, device=x.device)
+    arg = torch.empty((n, h // 2, w // 2, c), dtype=torch.uint8, device=x.device)
+    _check(_lib().ggh_pool2x2_forward(x.data_ptr(), y.data_ptr(), arg.data_ptr(), n, h, w, c, _stream(x)))
+    return y, arg
+
+
+def pool2x2_backward(dy, arg):
+    """the gradient at each window's first maximum, zero elsewhere"""
+    dy = _bf16(dy)
+    n, ho, wo, c = dy.shape
+    dx = torch.empty((n, 2 * ho, 2 * wo, c), dtype=torch.bfloat16, device=dy.device)
+    _check(_lib().ggh_pool2x2_backward(dy.data_ptr(), arg.data_ptr(), dx.data_ptr(), n, 2 * ho, 2 * wo, c,
+                                       _stream(dy)))
+    return dx
+
+
+def upcat(z, s):
+    """(z upsampled 2x by repetition, s) concatenated over channels, NHWC BF16"""
+    z, s = _bf16(z), _bf16(s)
+    n, h, w, cs = s.shape
+    cz = z.shape[3]
+    out = torch.empty((n, h, w, cz + cs), dtype=torch.bfloat16, device=s.device)
+    _check(_lib().ggh_upcat_forward(z.data_ptr(), s.data_ptr(), out.data_ptr(), n, h, w, cz, cs, _stream(s)))
+    return out
+
+
+def upcat_backward(dout, cz):
+    """(dz: the z channels of dout summed over each 2 x 2 block, ds: its s channels)"""
+    dout = _bf16(dout)
+    n, h, w, ct = dout.shape
+    dz = torch.empty((n, h // 2, w // 2, cz), dtype=torch.bfloat16, device=dout.device)
+    ds = torch.empty((n, h, w, ct - cz), dtype=torch.bfloat16, device=dout.device)
+    _check(_lib().ggh_upcat_backward(dout.data_ptr(), dz.data_ptr(), ds.data_ptr(), n, h, w, cz, ct - cz,
+                                     _stream(dout)))
+    return dz, ds
+
+
+class _Pool2x2(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x):
+        y, arg = pool2x2(x)
+        ctx.save_for_backward(arg)
+        return y
+
+    @staticmethod
+    def backward(ctx, dy):
+        (arg,) = ctx.saved_tensors
+        return pool2x2_backward(dy, arg)
+
+
+class _UpCat(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, z, s):
+        ctx.cz = z.shape[3]
+        return upcat(z, s)
+
+    @staticmethod
+    def backward(ctx, dout):
+        return upcat_backward(dout, ctx.cz)
 
 
 IN_CHANNELS = 8  # the two inputs, zero-padded to the kernels' channel multiple
@@ -203,9 +267,9 @@ class UNet(torch.nn.Module):
             z = b(a(z))
             if i + 1 < len(self.enc):
                 skips.append(z)
-                z = _pool(z)
+                z = _Pool2x2.apply(z)
         for (a, b), s in zip(self.dec, reversed(skips)):
-            z = b(a(torch.cat([_up(z), s], dim=3)))
+            z = b(a(_UpCat.apply(z, s)))
         # the 16 -> 1 head as a multiply and a channel sum: a matrix-vector
         # product with 16 columns runs far below HBM speed in the BLAS path
         return (z.float() * self.head).sum(dim=3) + self.head_bias

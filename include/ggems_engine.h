@@ -320,6 +320,23 @@ int ggh_conv3x3_weights_bf16(const float* w, void* w_bf16, void* wt_bf16, int32_
 int ggh_relu_bias_backward(const void* dy, int32_t dy_is_f32, const void* y, int32_t relu, void* dy_out, float* db,
                            int64_t n_pixels, int32_t c, void* stream);
 
+/* 2 x 2 max pooling, NHWC BF16 x [n][h][w][c] -> y [n][h/2][w/2][c]; arg
+ * (one byte per output element) keeps which input of the window (row-major,
+ * 0..3) held the first maximum.  h, w even; c % 8 == 0. */
+int ggh_pool2x2_forward(const void* x, void* y, uint8_t* arg, int32_t n, int32_t h, int32_t w_px, int32_t c,
+                        void* stream);
+/* dx [n][h][w][c] = dy at the input arg names, 0 at the other three */
+int ggh_pool2x2_backward(const void* dy, const uint8_t* arg, void* dx, int32_t n, int32_t h, int32_t w_px, int32_t c,
+                         void* stream);
+/* the decoder's input: out [n][h][w][cz + cs] = (z [n][h/2][w/2][cz] repeated
+ * over each 2 x 2 block, s [n][h][w][cs]); cz, cs % 8 == 0 */
+int ggh_upcat_forward(const void* z, const void* s, void* out, int32_t n, int32_t h, int32_t w_px, int32_t cz,
+                      int32_t cs, void* stream);
+/* its backward: dz = dout's first cz channels summed over each 2 x 2 block
+ * (FP32, rounded once), ds = dout's last cs channels */
+int ggh_upcat_backward(const void* dout, void* dz, void* ds, int32_t n, int32_t h, int32_t w_px, int32_t cz,
+                       int32_t cs, void* stream);
+
 /* ---- test entry points -------------------------------------------------- */
 /* voxel indices visited by rays crossing voxel solid `solid` with no
  * interaction; pos/dir are global.  out_index[r*max_steps + k], out_count[r]. */

@@ -115,3 +115,39 @@ def test_weights_bf16_equal_cast_and_flip_transpose(cout, cin):
     ref = w.to(torch.bfloat16)
     assert torch.equal(wb.view(torch.int16), ref.view(torch.int16))
     assert torch.equal(wt.view(torch.int16), S.flip_transpose(ref).view(torch.int16))
+
+
+@pytest.mark.parametrize("n,h,w,c", [(2, 32, 64, 16), (3, 16, 16, 128)])
+def test_pool2x2_matches_max_pool2d(n, h, w, c):
+    """values and gradients of max_pool2d on the CPU, bit for bit; a window
+    of ties (zeros, as after a ReLU) gives the gradient to its first input"""
+    g = torch.Generator().manual_seed(h + c)
+    x = torch.randn(n, h, w, c, generator=g).bfloat16()
+    x[0, :4, :4, :] = 0.0
+    xr = x.float().permute(0, 3, 1, 2).contiguous().requires_grad_(True)
+    ref = Fn.max_pool2d(xr, 2)
+    dy = torch.randn(ref.shape, generator=g).bfloat16()
+    ref.backward(dy.float())
+    y, arg = S.pool2x2(x.cuda())
+    assert torch.equal(y.cpu().float(), ref.detach().permute(0, 2, 3, 1))
+    dx = S.pool2x2_backward(dy.permute(0, 2, 3, 1).contiguous().cuda(), arg)
+    assert torch.equal(dx.cpu().float(), xr.grad.permute(0, 2, 3, 1))
+
+
+def test_upcat_matches_cat_of_nearest_upsampling():
+    """the decoder input against cat(interpolate(z, nearest), s) on the CPU,
+    and its gradients: ds exact, dz the FP32 sum rounded once to BF16"""
+    g = torch.Generator().manual_seed(5)
+    z = torch.randn(2, 8, 16, 32, generator=g).bfloat16()
+    s = torch.randn(2, 16, 32, 16, generator=g).bfloat16()
+    zr = z.float().requires_grad_(True)
+    sr = s.float().requires_grad_(True)
+    up = Fn.interpolate(zr.permute(0, 3, 1, 2), scale_factor=2, mode="nearest").permute(0, 2, 3, 1)
+    ref = torch.cat([up, sr], dim=3)
+    dout = torch.randn(ref.shape, generator=g).bfloat16()
+    ref.backward(dout.float())
+    out = S.upcat(z.cuda(), s.cuda())
+    assert torch.equal(out.cpu().float(), ref.detach())
+    dz, ds = S.upcat_backward(dout.cuda(), 32)
+    assert torch.equal(ds.cpu().float(), sr.grad)
+    assert torch.equal(dz.cpu(), zr.grad.bfloat16())
