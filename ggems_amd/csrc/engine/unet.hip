@@ -539,13 +539,17 @@ __global__ __launch_bounds__(kGlueBlock) void upcat_backward_kernel(const __bf16
 // reversed (tap 8 - t), in place of a cast, a flip and a transposing copy.
 constexpr int kPrepBlock = 256;
 __global__ __launch_bounds__(kPrepBlock) void weights_bf16_kernel(const float* __restrict__ w, __bf16* __restrict__ wb,
-                                                                   __bf16* __restrict__ wt, int cout, int cin) {
+                                                                   __bf16* __restrict__ wt, float* __restrict__ dw,
+                                                                   float* __restrict__ db, int cout, int cin) {
   const int n = cout * 9 * cin;
   for (int i = blockIdx.x * kPrepBlock + threadIdx.x; i < n; i += gridDim.x * kPrepBlock) {
     const int ci = i % cin, t = (i / cin) % 9, co = i / (9 * cin);
     const __bf16 v = (__bf16)w[i];
     wb[i] = v;
     wt[((size_t)ci * 9 + (8 - t)) * cout + co] = v;
+    // the layer's gradient accumulators, zeroed here instead of by fills of their own
+    if (dw) dw[i] = 0.0f;
+    if (db && i < cout) db[i] = 0.0f;
   }
 }
 
@@ -612,13 +616,14 @@ int ggh_relu_bias_backward(const void* dy, int32_t dy_is_f32, const void* y, int
   return 0;
 }
 
-int ggh_conv3x3_weights_bf16(const float* w, void* w_bf16, void* wt_bf16, int32_t cout, int32_t cin, void* stream) {
+int ggh_conv3x3_weights_bf16(const float* w, void* w_bf16, void* wt_bf16, float* dw_zero, float* db_zero,
+                             int32_t cout, int32_t cin, void* stream) {
   if (!w || !w_bf16 || !wt_bf16) return ufail(-1, "null pointer");
   if (cout < 1 || cin < 1 || (long long)cout * 9 * cin >= (1ll << 31)) return ufail(-2, "unsupported shape");
   const long long n = (long long)cout * 9 * cin;
   const unsigned blocks = (unsigned)((n + kPrepBlock - 1) / kPrepBlock < 4096 ? (n + kPrepBlock - 1) / kPrepBlock : 4096);
   hipLaunchKernelGGL(weights_bf16_kernel, dim3(blocks), dim3(kPrepBlock), 0, static_cast<hipStream_t>(stream), w,
-                     static_cast<__bf16*>(w_bf16), static_cast<__bf16*>(wt_bf16), cout, cin);
+                     static_cast<__bf16*>(w_bf16), static_cast<__bf16*>(wt_bf16), dw_zero, db_zero, cout, cin);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return ufail(-100 - (int)e, std::string("BF16 weights: ") + hipGetErrorString(e));
   return 0;

@@ -35,7 +35,7 @@ _SIGS = {
     "ggh_conv3x3_weight_grad": ([C.c_void_p, C.c_void_p, C.c_void_p] + [C.c_int32] * 6 + [C.c_void_p], C.c_int),
     "ggh_relu_bias_backward": ([C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64,
                                 C.c_int32, C.c_void_p], C.c_int),
-    "ggh_conv3x3_weights_bf16": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p], C.c_int),
+    "ggh_conv3x3_weights_bf16": ([C.c_void_p] * 5 + [C.c_int32, C.c_int32, C.c_void_p], C.c_int),
     "ggh_pool2x2_forward": ([C.c_void_p, C.c_void_p, C.c_void_p] + [C.c_int32] * 4 + [C.c_void_p], C.c_int),
     "ggh_pool2x2_backward": ([C.c_void_p, C.c_void_p, C.c_void_p] + [C.c_int32] * 4 + [C.c_void_p], C.c_int),
     "ggh_upcat_forward": ([C.c_void_p, C.c_void_p, C.c_void_p] + [C.c_int32] * 5 + [C.c_void_p], C.c_int),
@@ -79,12 +79,13 @@ def conv3x3(x, w, bias=None, relu=False):
     return y
 
 
-def conv3x3_weight_grad(x, dy, splits=None):
-    """sum over pixels of dy (x) the shifted x: [Cout, 9*Cin] FP32"""
+def conv3x3_weight_grad(x, dy, splits=None, out=None):
+    """sum over pixels of dy (x) the shifted x: [Cout, 9*Cin] FP32, added to
+    `out` when given (zeroed by the caller), else to a new zero tensor"""
     x, dy = x.contiguous(), dy.contiguous()
     n, h, wd, cin = x.shape
     cout = dy.shape[3]
-    dw = torch.zeros((cout, 9 * cin), dtype=torch.float32, device=x.device)
+    dw = torch.zeros((cout, 9 * cin), dtype=torch.float32, device=x.device) if out is None else out
     if splits is None:
         # pixel splits: about 1,024 pixels each, and at least 1,024 blocks in
         # all (cols x splits) so that the small deep layers fill the chip
@@ -96,16 +97,17 @@ def conv3x3_weight_grad(x, dy, splits=None):
     return dw
 
 
-def relu_bias_backward(dy, y, relu):
+def relu_bias_backward(dy, y, relu, db=None):
     """(dy masked where the layer's output y <= 0 when relu, as BF16; the bias
-    gradient, the sum of that over the pixels in FP32) in one kernel"""
+    gradient, the sum of that over the pixels in FP32, added to `db` when
+    given (zeroed by the caller)) in one kernel"""
     if dy.dtype not in (torch.bfloat16, torch.float32):
         dy = dy.float()
     dy = dy.contiguous()
     c = dy.shape[-1]
     yc = y.contiguous() if relu else None
     out = torch.empty(dy.shape, dtype=torch.bfloat16, device=dy.device)
-    db = torch.zeros(c, dtype=torch.float32, device=dy.device)
+    db = torch.zeros(c, dtype=torch.float32, device=dy.device) if db is None else db
     _check(_lib().ggh_relu_bias_backward(dy.data_ptr(), int(dy.dtype == torch.float32),
                                          yc.data_ptr() if relu else None, int(bool(relu)),
                                          out.data_ptr(), db.data_ptr(), dy.numel() // c, c, _stream(dy)))
@@ -118,14 +120,18 @@ def flip_transpose(w):
     return w.flip(1).permute(2, 1, 0).contiguous()
 
 
-def weights_bf16(w):
+def weights_bf16(w, zero=None):
     """(w as BF16 [cout][9][cin], flip_transpose of it [cin][9][cout]) from the
-    FP32 weights in one kernel"""
+    FP32 weights in one kernel, which also sets the FP32 buffers `zero` =
+    (dw [cout, 9 cin], db [cout]) to zero when given"""
     cout, _, cin = w.shape
     w = w.detach().contiguous().float()
     wb = torch.empty((cout, 9, cin), dtype=torch.bfloat16, device=w.device)
     wt = torch.empty((cin, 9, cout), dtype=torch.bfloat16, device=w.device)
-    _check(_lib().ggh_conv3x3_weights_bf16(w.data_ptr(), wb.data_ptr(), wt.data_ptr(), cout, cin, _stream(w)))
+    dw, db = zero if zero is not None else (None, None)
+    _check(_lib().ggh_conv3x3_weights_bf16(w.data_ptr(), wb.data_ptr(), wt.data_ptr(),
+                                           None if dw is None else dw.data_ptr(), None if db is None else db.data_ptr(),
+                                           cout, cin, _stream(w)))
     return wb, wt
 
 
@@ -133,22 +139,30 @@ class _Conv3x3(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x, w, b, relu):
         cout, _, cin = w.shape
-        wb, wt = weights_bf16(w)
+        # the gradient accumulators, zeroed by the weight preparation's pass
+        zero = None
+        if ctx.needs_input_grad[1] or ctx.needs_input_grad[2]:
+            zero = (torch.empty((cout, 9 * cin), dtype=torch.float32, device=w.device),
+                    torch.empty(cout, dtype=torch.float32, device=w.device))
+        wb, wt = weights_bf16(w, zero)
         y = conv3x3(x, wb.view(cout, 9 * cin), b, relu)
         ctx.save_for_backward(x, wt, y)
         ctx.relu = relu
         ctx.shape = (cout, cin)
+        ctx.zero = zero
         return y
 
     @staticmethod
     def backward(ctx, dy):
         x, wt, y = ctx.saved_tensors
         cout, cin = ctx.shape
-        dy, db = relu_bias_backward(dy, y, ctx.relu)
+        dw_buf, db_buf = ctx.zero if ctx.zero is not None else (None, None)
+        ctx.zero = None  # a second reference would make autograd copy the gradients instead of taking them
+        dy, db = relu_bias_backward(dy, y, ctx.relu, db=db_buf)
         dx = None
         if ctx.needs_input_grad[0]:
             dx = conv3x3(dy, wt.view(cin, 9 * cout))
-        dw = conv3x3_weight_grad(x, dy).view(cout, 9, cin)
+        dw = conv3x3_weight_grad(x, dy, out=dw_buf).view(cout, 9, cin)
         return dx, dw, db, None
 
 

@@ -311,8 +311,11 @@ int ggh_conv3x3_weight_grad(const void* x, const void* dy, float* dw, int32_t n,
 
 /* a layer's BF16 operands from its FP32 weights w [cout][9][cin]: w_bf16
  * [cout][9][cin] (round to nearest even) and, for the data gradient, wt_bf16
- * [cin][9][cout] with the taps reversed (tap 8 - t) */
-int ggh_conv3x3_weights_bf16(const float* w, void* w_bf16, void* wt_bf16, int32_t cout, int32_t cin, void* stream);
+ * [cin][9][cout] with the taps reversed (tap 8 - t).  The same pass sets the
+ * layer's FP32 gradient accumulators dw_zero [cout][9][cin] and db_zero
+ * [cout] to zero; either may be null. */
+int ggh_conv3x3_weights_bf16(const float* w, void* w_bf16, void* wt_bf16, float* dw_zero, float* db_zero,
+                             int32_t cout, int32_t cin, void* stream);
 
 /* backward of a layer's ReLU and bias: dy_out = dy where y > 0 (everywhere
  * when relu == 0), BF16; db[c] += sum over the pixels of dy_out, FP32.

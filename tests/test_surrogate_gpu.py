@@ -109,12 +109,18 @@ def test_training_steps_reduce_the_loss():
 @pytest.mark.parametrize("cout,cin", [(16, 8), (64, 192), (128, 128)])
 def test_weights_bf16_equal_cast_and_flip_transpose(cout, cin):
     """the fused weight preparation equals torch's round-to-nearest cast and
-    the data gradient's flipped, transposed weights, bit for bit"""
+    the data gradient's flipped, transposed weights, bit for bit, and sets
+    the layer's gradient accumulators to zero"""
     w = torch.randn(cout, 9, cin, device="cuda") * 0.1
     wb, wt = S.weights_bf16(w)
     ref = w.to(torch.bfloat16)
     assert torch.equal(wb.view(torch.int16), ref.view(torch.int16))
     assert torch.equal(wt.view(torch.int16), S.flip_transpose(ref).view(torch.int16))
+    dw = torch.full((cout, 9 * cin), 7.0, device="cuda")
+    db = torch.full((cout,), 7.0, device="cuda")
+    wb2, _ = S.weights_bf16(w, (dw, db))
+    assert torch.equal(wb2, wb)
+    assert not dw.any() and not db.any()
 
 
 @pytest.mark.parametrize("n,h,w,c", [(2, 32, 64, 16), (3, 16, 16, 128)])
