@@ -3038,23 +3038,38 @@ int ggh_autotune(ggh_context* c, uint64_t n_trial, ggh_launch_desc* best) {
             keep_blocks = c->blocks_per_cu;
   float best_ms = INFINITY;
   ggh_launch_desc b{};
-  for (int blocks : {1, 2})
-    for (int round : {12, 20, 32})
-      for (int batch : {8, 16, 32}) {
+  // With the reference sampling the grid is round steps x interaction
+  // batch.  Tau sampling leaves its interactions to the navigation passes,
+  // so the batch does not apply, and it keeps its passes per round (one
+  // pass beat two at full size by less than a short trial's timing noise):
+  // its grid is round steps only, centred on its measured best.
+  const bool tau = c->tau_mode != 0;
+  const int rounds[2][4] = {{12, 20, 32, 0}, {12, 16, 20, 24}};
+  const int third[2][3] = {{8, 16, 32}, {0, 0, 0}};
+  for (int blocks : {1, 2}) {
+    for (int round : rounds[tau]) {
+      if (round == 0) continue;
+      for (int t : third[tau]) {
+        if (tau && t != third[tau][0]) continue;  // one trial per round length
         c->blocks_per_cu = blocks;
         c->round_steps = round;
-        c->interact_batch = batch;
+        c->interact_batch = tau ? keep_batch : t;
+        c->cold_passes = keep_cold;
         HIP_TRY(hipMemcpyAsync(c->rng.p, saved.p, saved.n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
         if ((rc = ggh_run(c, n_trial))) break;
         if (c->last_ms < best_ms) {
           best_ms = c->last_ms;
           b.round_steps = round;
-          b.cold_passes = keep_cold;
-          b.interact_batch = batch;
+          b.cold_passes = c->cold_passes;
+          b.interact_batch = c->interact_batch;
           b.blocks_per_cu = blocks;
           b.general_kernel = c->force_general ? 1 : 0;
         }
       }
+      if (rc) break;  // a failed trial ends the search with its error
+    }
+    if (rc) break;
+  }
   const hipError_t e = hipMemcpyAsync(c->rng.p, saved.p, saved.n * sizeof(uint32_t), hipMemcpyDeviceToDevice,
                                       c->stream);
   (void)hipStreamSynchronize(c->stream);

@@ -878,5 +878,6 @@ void GGEMS::autotune(uint64_t n_trial) {
   ggh_launch_desc best;
   engine_check(ggh_autotune(ctx, n_trial, &best), "autotune");
   log(1, "GGEMS", "autotuned launch: round steps " + std::to_string(best.round_steps) + ", interaction batch " +
-                      std::to_string(best.interact_batch) + ", blocks per CU " + std::to_string(best.blocks_per_cu));
+                      std::to_string(best.interact_batch) + ", navigation passes " +
+                      std::to_string(best.cold_passes) + ", blocks per CU " + std::to_string(best.blocks_per_cu));
 }

@@ -261,8 +261,8 @@ int ggh_drr(ggh_context* ctx, int solid, int projection, float* image);
 int ggh_drr_noalloc(ggh_context* ctx, int solid, int projection, float* image, void* stream);
 
 /* Launch autotuner: times ggh_run(n_trial) over a grid of launch shapes
- * (blocks per compute unit, round steps, interaction batch) and keeps the
- * fastest.  Call after ggh_seed and before any run: it restores the seeded
+ * (blocks per compute unit, round steps, and with the reference sampling the
+ * interaction batch) and keeps the fastest.  Call after ggh_seed and before any run: it restores the seeded
  * random streams and zeroes every tally and counter afterwards. */
 int ggh_autotune(ggh_context* ctx, uint64_t n_trial, ggh_launch_desc* best);
 int ggh_get_dose_tallies(ggh_context* ctx, double* edep, double* edep_squared, int32_t* hit,

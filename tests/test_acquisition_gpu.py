@@ -112,6 +112,31 @@ def test_autotune_restores_streams_and_tallies():
     fresh.close()
 
 
+def test_autotune_tau_sampling_tunes_round_length():
+    """with tau sampling the grid is blocks x round steps around its best,
+    the interaction batch and the navigation passes keep their values, and
+    the streams are restored"""
+    sc = _scene()
+    sc.tau = True
+    tuned = sc.make_engine()
+    tuned.seed(43)
+    best = tuned.autotune(2_000_000)
+    assert best["blocks_per_cu"] in (1, 2) and best["round_steps"] in (12, 16, 20, 24)
+    assert best["cold_passes"] == 0 and best["interact_batch"] == 16
+    assert tuned.stats()["primaries"] == 0
+    tuned.reset_tallies()
+    tuned.run(300_000)
+    h_t, _ = tuned.histogram(sc.engine_solids["box"][0])
+    fresh = sc.make_engine()
+    fresh.seed(43)
+    fresh.reset_tallies()
+    fresh.run(300_000)
+    h_f, _ = fresh.histogram(sc.engine_solids["box"][0])
+    assert np.array_equal(h_t, h_f)
+    tuned.close()
+    fresh.close()
+
+
 def test_acquisition_script_matches_oracle(tmp_path):
     """configs.acquisition through the GGEMS API: the saved projection stack
     against the oracle's consecutive runs on the same inputs"""
