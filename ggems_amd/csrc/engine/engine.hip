@@ -3045,15 +3045,14 @@ int ggh_autotune(ggh_context* c, uint64_t n_trial, ggh_launch_desc* best) {
   // its grid is round steps only, centred on its measured best.
   const bool tau = c->tau_mode != 0;
   const int rounds[2][4] = {{12, 20, 32, 0}, {12, 16, 20, 24}};
-  const int third[2][3] = {{8, 16, 32}, {0, 0, 0}};
+  const int batches[3] = {8, 16, 32};
   for (int blocks : {1, 2}) {
     for (int round : rounds[tau]) {
       if (round == 0) continue;
-      for (int t : third[tau]) {
-        if (tau && t != third[tau][0]) continue;  // one trial per round length
+      for (int bi = 0; bi < (tau ? 1 : 3); ++bi) {  // tau sampling: one trial per round length
         c->blocks_per_cu = blocks;
         c->round_steps = round;
-        c->interact_batch = tau ? keep_batch : t;
+        c->interact_batch = tau ? keep_batch : batches[bi];
         c->cold_passes = keep_cold;
         HIP_TRY(hipMemcpyAsync(c->rng.p, saved.p, saved.n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
         if ((rc = ggh_run(c, n_trial))) break;
