@@ -186,3 +186,27 @@ def test_tau_against_reference_sampling_at_full_size(body):
     for k in ("voxel_steps", "interactions", "detector_interactions", "hops"):
         print("[stat] %s per primary: reference %.6g, tau %.6g, null %.6g" % (k, rs[k] / n, ts[k] / n, ns[k] / n))
         assert abs(ts[k] - rs[k]) <= max(6 * abs(ns[k] - rs[k]), 2e-3 * rs[k]), (k, rs[k], ts[k], ns[k])
+
+
+def test_tau_against_reference_sampling_at_1e8(body):
+    """10^8 primaries each (ten times the test above): tau sampling (seed
+    3001) against the reference sampling (seed 3002), the reference against
+    itself (seed 3003) as the null; dose in 8^3-dosel blocks, the projection
+    in 4x4-pixel blocks, and the per-primary counters within their
+    statistical spread."""
+    n = 100_000_000
+    eng = body.make_engine()
+    ref = body.run_engine(n, 3002, eng)
+    null = body.run_engine(n, 3003, eng)
+    eng.set_sampling("tau")
+    tau = body.run_engine(n, 3001, eng)
+    eng.close()
+    parity.same_distribution("dose blocks 8^3 at 1e8, tau vs reference sampling (null: reference vs reference)",
+                             parity.dose_z(tau, ref), parity.dose_z(null, ref), 20000)
+    parity.same_distribution("projection 4x4 pixel blocks at 1e8, tau vs reference sampling (null: reference vs "
+                             "reference)", parity.pixel_z(tau, ref, b=4), parity.pixel_z(null, ref, b=4), 2000)
+    rs, ts, ns = ref["stats"], tau["stats"], null["stats"]
+    for k in ("voxel_steps", "interactions", "detector_interactions", "hops"):
+        print("[stat] %s per primary at 1e8: reference %.7g, tau %.7g, null %.7g"
+              % (k, rs[k] / n, ts[k] / n, ns[k] / n))
+        assert abs(ts[k] - rs[k]) <= max(6 * abs(ns[k] - rs[k]), 5e-4 * rs[k]), (k, rs[k], ts[k], ns[k])
