@@ -603,7 +603,11 @@ int ggh_relu_bias_backward(const void* dy, int32_t dy_is_f32, const void* y, int
     return ufail(-2, "unsupported shape (C one of 8, 16, 32, 64, 128)");
   const long long pieces = (long long)n_pixels * (c / 8);
   const long long want = (pieces + kMaskBlock - 1) / kMaskBlock;
-  const unsigned blocks = (unsigned)(want < 1024 ? want : 1024);
+  // one block per compute unit: every block ends with one global atomic per
+  // channel onto the same C addresses, and more blocks queue up there
+  // (measured over a training run: 92.7 ms at 256 blocks, 100 at 512, 143 at
+  // 1,024, 294 at 4,096, 125 at 128)
+  const unsigned blocks = (unsigned)(want < 256 ? want : 256);
   const hipStream_t st = static_cast<hipStream_t>(stream);
   if (dy_is_f32)
     hipLaunchKernelGGL(relu_bias_backward_kernel<true>, dim3(blocks), dim3(kMaskBlock), 0, st, dy,
