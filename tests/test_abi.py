@@ -99,3 +99,42 @@ def test_engine_reports_errors_without_a_device():
     lib = C.CDLL(os.path.join(LIBDIR, "libggems_engine.so"))
     lib.ggh_last_error.restype = C.c_char_p
     assert isinstance(lib.ggh_last_error(), bytes)
+
+
+def test_unet_entry_points_validate_before_touching_the_device():
+    """the U-Net's C entry points refuse null pointers (-1) and shapes their
+    kernels do not take (-2), with a message, before any device call: the
+    pointers below are never dereferenced"""
+    lib = C.CDLL(os.path.join(LIBDIR, "libggems_engine.so"))
+    lib.ggh_unet_last_error.restype = C.c_char_p
+    p = C.c_void_p(256)
+    i = C.c_int32
+    sigs = {
+        "ggh_conv3x3_forward": [C.c_void_p] * 4 + [i] * 6 + [C.c_void_p],
+        "ggh_conv3x3_weight_grad": [C.c_void_p] * 3 + [i] * 6 + [C.c_void_p],
+        "ggh_relu_bias_backward": [C.c_void_p, i, C.c_void_p, i, C.c_void_p, C.c_void_p, C.c_int64, i, C.c_void_p],
+        "ggh_pool2x2_forward": [C.c_void_p] * 3 + [i] * 4 + [C.c_void_p],
+        "ggh_pool2x2_backward": [C.c_void_p] * 3 + [i] * 4 + [C.c_void_p],
+        "ggh_upcat_forward": [C.c_void_p] * 3 + [i] * 5 + [C.c_void_p],
+        "ggh_upcat_backward": [C.c_void_p] * 3 + [i] * 5 + [C.c_void_p],
+    }
+    for name, args in sigs.items():
+        getattr(lib, name).argtypes = args
+        getattr(lib, name).restype = C.c_int
+    cases = [
+        (lib.ggh_conv3x3_forward(None, p, None, p, 1, 32, 32, 8, 16, 0, None), -1),
+        (lib.ggh_conv3x3_forward(p, p, None, p, 1, 32, 24, 8, 16, 0, None), -2),     # W % 16
+        (lib.ggh_conv3x3_forward(p, p, None, p, 1, 32, 32, 12, 16, 0, None), -2),    # Cin % 8
+        (lib.ggh_conv3x3_weight_grad(p, p, p, 1, 32, 32, 8, 256, 0, None), -2),      # Cout > 128
+        (lib.ggh_relu_bias_backward(p, 0, p, 1, p, p, 100, 24, None), -2),           # C not a power of two
+        (lib.ggh_relu_bias_backward(p, 0, None, 1, p, p, 100, 16, None), -1),        # ReLU without y
+        (lib.ggh_pool2x2_forward(p, p, None, 1, 4, 4, 8, None), -1),
+        (lib.ggh_pool2x2_forward(p, p, p, 1, 3, 4, 8, None), -2),                    # odd height
+        (lib.ggh_pool2x2_backward(p, p, p, 1, 4, 4, 12, None), -2),                  # C % 8
+        (lib.ggh_upcat_forward(p, p, p, 1, 4, 6, 8, 12, None), -2),                  # Cs % 8
+        (lib.ggh_upcat_backward(p, None, p, 1, 4, 4, 8, 8, None), -1),
+        (lib.ggh_upcat_backward(p, p, p, 1, 4, 5, 8, 8, None), -2),                  # odd width
+    ]
+    for got, want in cases:
+        assert got == want, (got, want)
+        assert lib.ggh_unet_last_error()
